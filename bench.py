@@ -1,0 +1,153 @@
+"""Benchmark of the density-peak hot path (kNN -> rho -> delta/nneigh) on one MI355X.
+
+A step is one irc_rho_delta_batch call over the whole drift-10M workload (384 sites,
+10M spikes, D=42, 200 drift chunks, 15 links within +/-64, k=30) with features resident in
+HBM.  `value` is spikes/s of the headline form (BF16 screen + FP64 re-rank); the plain
+FP32-screen form is timed in the same run ("forms").  The CPU baseline is the FP32
+restatement of the reference CPU path timed on a bounded sample of sites.
+
+  python bench.py [--steps K] [--warmup W] [--workload drift-10M] [--forms bf16,fp32]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spikes/s via kNN+rho+delta at 10M spikes; BF16-screen vs FP32, exact kNN"
+PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "exact": 78.6e12}
+PREC = {"exact": 0, "fp32": 1, "bf16": 2}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="drift-10M")
+    ap.add_argument("--forms", default="bf16,fp32")
+    ap.add_argument("--cpu-sites", type=int, default=0, help="sites in the CPU baseline sample (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", action="store_true", help="verify size-independent properties of the output")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(the path would shard by site, see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from ironclust_amd import irc2, libirc, synth
+
+    t0 = time.time()
+    ds = synth.make_workload(a.workload, device="cuda")
+    lay = irc2.site_layout(ds)
+    torch.cuda.synchronize()
+    N, k, D = ds["n_spikes"], ds["k"], ds["D"]
+    pairs = synth.n_pairs(ds)
+    print(f"# data {a.workload}: N={N} D={D} sites={ds['n_sites']} chunks={ds['n_drift']} "
+          f"pairs/pass={pairs:.4e} gen {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"],
+               n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+    ctx = libirc.Context(0)
+    forms = {}
+    results = {}
+    for form in a.forms.split(","):
+        opts = libirc.make_options(precision=PREC[form])
+        for _ in range(a.warmup):
+            ctx.rho_delta_batch_device(dev, out, opts)
+        torch.cuda.synchronize()
+        st_acc = {"ms_knn": 0.0, "ms_delta": 0.0, "ms_total": 0.0, "ms_screen": 0.0}
+        t1 = time.perf_counter()
+        for _ in range(a.steps):
+            ctx.rho_delta_batch_device(dev, out, opts)
+            s = ctx.stats()
+            for kk in st_acc:
+                st_acc[kk] += s.get(kk, 0.0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / a.steps
+        s = ctx.stats()
+        ms_kernel = st_acc["ms_screen"] / a.steps if st_acc["ms_screen"] > 0 else st_acc["ms_knn"] / a.steps
+        flops = 2.0 * 2 * D * pairs / 2          # one pass (kNN) of 2*D flops per pair
+        forms[form] = {
+            "spikes_per_s": N / dt, "ms_per_step": dt * 1e3,
+            "ms_knn": st_acc["ms_knn"] / a.steps, "ms_delta": st_acc["ms_delta"] / a.steps,
+            "ms_device_total": st_acc["ms_total"] / a.steps,
+            "knn_kernel_ms": ms_kernel,
+            "knn_tflops": flops / (ms_kernel * 1e-3) / 1e12,
+            "screen_fallback": int(s["n_screen_fallback"]), "delta_full_scan": int(s["n_delta_full"]),
+            "precision_used": int(s["precision_used"]),
+        }
+        results[form] = {kk: v.clone() for kk, v in out.items()}
+        print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
+    head = a.forms.split(",")[0]
+    # all forms must agree bit for bit
+    for form in results:
+        for kk in results[head]:
+            assert torch.equal(results[form][kk], results[head][kk]), (form, kk)
+    if a.check:
+        r = results[head]
+        ids = torch.arange(1, N + 1, device="cuda")
+        has = r["nneigh"] != ids
+        assert bool((r["rho"][r["nneigh"][has] - 1] > r["rho"][has]).all())
+        assert bool((r["knn"][:, 0] == ids).all())
+    hf = forms[head]
+    flops = 2.0 * D * pairs
+    peak = PEAK[head]
+    ach = flops / (hf["knn_kernel_ms"] * 1e-3)
+    line = {
+        "metric": METRIC, "value": hf["spikes_per_s"], "unit": "spikes/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": hf["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"bf16": "bf16", "fp32": "fp32", "exact": "f64"}[head],
+        "data": "synthetic: seeded drift recording generated on the GPU, features resident in HBM",
+        "config": {"workload": a.workload, "n_spikes": N, "n_sites": ds["n_sites"], "D": D,
+                   "n_drift": ds["n_drift"], "k": k, "pairs_per_pass": pairs,
+                   "form": {"bf16": "BF16 hi/lo MFMA screen + FP64 re-rank", "fp32": "FP32 MFMA screen + FP64 re-rank",
+                            "exact": "FP64 exact"}[head]},
+        "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": ach / peak, "traffic": None,
+                     "kernel": "kNN pass (screen kernel when screened)",
+                     "flops_per_launch": flops},
+        "forms": forms,
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(ds, a)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(ds, a):
+    """FP32 restatement of knn_cpu_ + delta_cpu_ (OpenMP over sites) on a bounded sample of
+    sites of the same workload; spikes/s = sampled spikes / wall time."""
+    import numpy as np
+    from oracle import oracle as O
+    host = dict(ds)
+    for kk in ("F1", "site1"):
+        host[kk] = ds[kk].cpu().numpy()
+    host["F2"] = None if ds.get("F2") is None else ds["F2"].cpu().numpy()
+    host["site2"] = None if ds.get("site2") is None else ds["site2"].cpu().numpy()
+    lib = O.cpu_port()
+    cores = int(lib.cpu_max_threads())
+    S = ds["n_sites"]
+    n_sites = a.cpu_sites or max(1, min(S, cores))
+    sites = np.arange(1, n_sites + 1, dtype=np.int32)
+    n_spk = int(np.isin(host["site1"], sites).sum())
+    t = time.perf_counter()
+    O.cpu_sort_batch(host, sites=sites, nthreads=cores)
+    dt = time.perf_counter() - t
+    return {"value": n_spk / dt, "unit": "spikes/s", "cores": cores, "kind": "port",
+            "sample": f"{n_sites} of {S} sites ({n_spk} spikes) of {a.workload}, {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

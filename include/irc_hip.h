@@ -1,0 +1,213 @@
+/*
+ * irc_hip.h -- C ABI of the MI355X (gfx950) density-peak hot path of IronClust:
+ * drift-constrained per-site kNN, rho = 1/d_k, and delta/nneigh (nearest denser
+ * spike within the linked drift chunks).
+ *
+ * The entry points replace the parallel.gpu.CUDAKernel/PTX launches of the
+ * reference (paths relative to the ironclust repository):
+ *
+ *   irc_knn_drift       search_knn_drift_   matlab/irc2.m:3731-3789  (kernel search_min_drift.cu:24-94)
+ *   irc_delta_drift     search_delta_drift_ matlab/irc2.m:3794-3865  (kernel search_delta_drift.cu:24-106)
+ *   irc_rho_delta_batch sort_page_          matlab/irc2.m:3456-3503  (rho_page_ 3524, delta_page_ 3566,
+ *                                            rho_paged_site_ 3635, delta_paged_site_ 3684), all sites and
+ *                                            drift chunks in one call
+ *   irc_knn_ab          cuda_knn_           matlab/irc2.m:4202-4238, irc.m:24365-24401
+ *                                            (kernels cuda_knn_index.cu:27-111, cuda_knn.cu, cuda_knn_full.cu,
+ *                                            cuda_knn_ib.cu)
+ *   irc_delta_knn_ab    cuda_delta_knn_     irc.m:24519-24553        (kernel cuda_delta_knn.cu:19-88)
+ *   irc_rho_drift_dc    cuda_rho_drift_     irc.m:24612-24637        (kernel irc_cuda_rho_drift.cu:28-108)
+ *   irc_delta_drift_dc  cuda_delta_drift_   irc.m:24641-24666        (kernel irc_cuda_delta_drift.cu:28-132)
+ *
+ * Results are exact to the canonical definition (FP64 distances accumulated with
+ * fused multiply-add in dimension order; ties broken by lower spike id), whatever
+ * precision option is chosen: a reduced-precision screen only selects candidates,
+ * and every reported distance, index, rho and delta comes from the FP64 re-rank.
+ *
+ * Conventions: features are D x n single precision, column-major (MATLAB layout),
+ * i.e. n consecutive groups of D floats.  Indices returned to the caller are 1-based.
+ * All calls are synchronous.  No C++ exception crosses this boundary.
+ * One context per host thread; the library is re-entrant across contexts.
+ */
+#ifndef IRC_HIP_H
+#define IRC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct irc_ctx irc_ctx;
+
+typedef enum {
+    IRC_OK = 0,
+    IRC_E_ARG = 1,          /* invalid argument (sizes, null pointers, out-of-range indices) */
+    IRC_E_NOMEM = 2,        /* device or host allocation failed */
+    IRC_E_HIP = 3,          /* HIP runtime error */
+    IRC_E_UNSUPPORTED = 4,  /* valid but unsupported (e.g. D too large) */
+    IRC_E_FORCED = 5        /* failure forced through irc_options.force_fail (test hook) */
+} irc_status;
+
+typedef enum {
+    IRC_PREC_EXACT = 0,        /* FP64 over every candidate pair */
+    IRC_PREC_FP32 = 1,         /* FP32 MFMA distance screen + FP64 re-rank */
+    IRC_PREC_BF16_SCREEN = 2,  /* BF16 (hi/lo split) MFMA screen, FP32 accumulate, + FP64 re-rank */
+    IRC_PREC_AUTO = 3          /* BF16 screen where supported, else FP32 */
+} irc_precision;
+
+typedef enum {
+    IRC_MEM_HOST = 0,    /* pointers in irc_batch / irc_outputs are host memory */
+    IRC_MEM_DEVICE = 1   /* pointers are device memory of the context's device */
+} irc_mem;
+
+typedef struct {
+    int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
+    int32_t k_over;     /* over-fetched candidates per query for the screen (0 = default 32) */
+    int32_t tile;       /* reserved for the tuner (0 = tuned default) */
+    int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
+    int32_t mem;        /* irc_mem for irc_rho_delta_batch */
+    int32_t flags;      /* bit 0: per-(site,chunk) launches instead of the batched launch */
+} irc_options;
+
+/* Fill *opt with defaults. */
+void irc_options_default(irc_options* opt);
+
+/* Create / destroy a context on HIP device `device` (ordinal).  The context owns
+ * device buffers, the stream and tuning state, and keeps them across calls
+ * (the role of the `persistent CK` kernel objects, matlab/irc2.m:3732). */
+irc_status irc_ctx_create(irc_ctx** ctx, int device);
+void irc_ctx_destroy(irc_ctx* ctx);
+const char* irc_last_error(const irc_ctx* ctx);
+
+/* Library / build information string (static storage). */
+const char* irc_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Per-site problem: one site, one page (prepare_page_site_, matlab/irc2.m:3869-3910).
+ *   F            mrFet_out, D x n_cols single (columns: primary spikes by ascending id, then
+ *                secondary spikes by ascending id)
+ *   drift_of_col viDrift_out, 1-based drift chunk of each column (1..n_drift)
+ *   id_of_col    optional global spike id per column, used only to break distance ties
+ *                (NULL: column index)
+ *   link         mlDrift1, n_drift x n_drift column-major: link[r + c*n_drift] != 0 when
+ *                chunk r is a candidate chunk for queries in chunk c
+ *   in_begin     0-based first query column (find(vl_in,1)-1); n_in query columns follow
+ */
+typedef struct {
+    int32_t D;
+    int32_t k;
+    int64_t n_cols;
+    const float* F;
+    const int32_t* drift_of_col;
+    const int64_t* id_of_col;
+    int32_t n_drift;
+    const uint8_t* link;
+    int64_t in_begin;
+    int64_t n_in;
+} irc_site_problem;
+
+/* search_knn_drift_: dk_out[n_in] = sqrt(d2 of the k-th neighbour) (vrKnn_in),
+ * knn_out[k x n_in] = 1-based column indices of the k nearest (miKnn_in), nearest first.
+ * Host memory. */
+irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_options* opt,
+                         float* dk_out, int32_t* knn_out);
+
+/* search_delta_drift_: rho_cols[n_cols] = rho per column (vrRho_out);
+ * dmin_out[n_in] = sqrt(d2) to the nearest column with larger rho (vrDelta_in, not yet
+ * multiplied by rho), +Inf when there is none; nneigh_out[n_in] = its 1-based column
+ * (the query's own column when there is none).  Host memory. */
+irc_status irc_delta_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_options* opt,
+                           const float* rho_cols, float* dmin_out, int32_t* nneigh_out);
+
+/* ---------------------------------------------------------------------------
+ * Batch problem: every site and drift chunk of a recording (sort_page_).
+ *   F            D x n_cols single, grouped by site: site s owns columns
+ *                [site_ptr[s], site_ptr[s+1]); its first site_n_primary[s] columns are the
+ *                spikes whose primary site is s (features at s, ascending id), the rest are
+ *                spikes whose secondary site is s (features at s, ascending id)
+ *                (the layout of _fet_/_fet2_ files, matlab/irc2.m:4583-4618, 4039-4135)
+ *   col_spike    1-based spike id of each column (int32; n_spikes < 2^31)
+ *   lim_drift    viLim_drift, n_drift+1 entries, 1-based: chunk c holds ids
+ *                lim_drift[c] .. lim_drift[c+1]-1 (matlab/irc2.m:6395)
+ *   link         mlDrift, n_drift x n_drift column-major (matlab/irc2.m:6433-6437)
+ * site_ptr / site_n_primary / lim_drift / link are always host memory; F and col_spike
+ * follow irc_options.mem.
+ */
+typedef struct {
+    int64_t n_spikes;
+    int32_t D;
+    int32_t k;
+    int32_t n_sites;
+    int32_t n_drift;
+    int64_t n_cols;
+    const float* F;
+    const int32_t* col_spike;
+    const int64_t* site_ptr;
+    const int64_t* site_n_primary;
+    const int64_t* lim_drift;
+    const uint8_t* link;
+} irc_batch;
+
+/* Outputs indexed by spike id - 1 (irc_options.mem selects host or device memory).
+ *   rho     float32(1/sqrt(d2_k))                    (vrRho before rho/max(rho), irc2.m:3669)
+ *   delta   float32(sqrt(d2_min) * rho)              (vrDelta, irc2.m:3720-3725)
+ *   nneigh  1-based spike id of the nearest denser spike, or self
+ *   knn     k x n_spikes, 1-based spike ids, nearest first (may be NULL) (_knn_<site>.irc)
+ *   dk      sqrt(d2_k) (may be NULL)
+ */
+typedef struct {
+    float* rho;
+    float* delta;
+    int64_t* nneigh;
+    int64_t* knn;
+    float* dk;
+} irc_outputs;
+
+irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* batch, const irc_options* opt,
+                               irc_outputs* out);
+
+/* Statistics of the last irc_rho_delta_batch / irc_knn_drift call (for benchmarks and tests). */
+typedef struct {
+    int64_t n_queries;
+    int64_t n_pairs;          /* sum over queries of |C(a)| */
+    int64_t n_screen_fallback;/* queries whose screened candidate set could not be certified */
+    int64_t n_delta_full;     /* queries whose nearest denser spike was not in their kNN list */
+    double ms_total;          /* device time of the whole call, HIP events */
+    double ms_knn;            /* kNN + rho kernels */
+    double ms_delta;          /* delta kernels */
+    int32_t precision_used;
+} irc_stats;
+
+irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);
+
+/* ---------------------------------------------------------------------------
+ * Legacy A/B forms (irc v4 path).
+ * irc_knn_ab: B = candidates (D x nB), A = queries (D x nA).  K_out[nA] = sqrt(d2_k);
+ * I_out[k x nA] = 1-based indices into B (NULL: not returned; the cuda_knn variant).
+ * Exact, ties by lower B index; d = 0 included (cuda_knn_index.cu:27-111 semantics made exact).
+ */
+irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A,
+                      int32_t k, float* K_out, int32_t* I_out);
+
+/* irc_delta_knn_ab: min distance from each A to B with R_B > R_A (cuda_delta_knn.cu:19-88);
+ * D_out = sqrt(d2_min) (+Inf when none), N_out 1-based index into B (0 when none). */
+irc_status irc_delta_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, const float* RB,
+                            int64_t nA, const float* A, const float* RA, float* D_out, int32_t* N_out);
+
+/* irc_rho_drift_dc / irc_delta_drift_dc: dc-count density and delta over one drift-chunk
+ * group with the primary/secondary site match rule (irc_cuda_rho_drift.cu:81-88).
+ *   miSite12  n12 x 2 column-major (primary site column, secondary site column)
+ *   fet1/fet2 D x n12 single; queries are the first n1 spikes
+ * rho_out[n1] = #(d2 <= dc2) / #compared; delta_out[n1] = sqrt(d2_min/dc2),
+ * nneigh_out[n1] 1-based (self when none). */
+irc_status irc_rho_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, const int32_t* miSite12,
+                            const float* fet1, const float* fet2, float dc2, float* rho_out);
+irc_status irc_delta_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, const int32_t* miSite12,
+                              const float* fet1, const float* fet2, const float* rho12, float dc2,
+                              float* delta_out, uint32_t* nneigh_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRC_HIP_H */

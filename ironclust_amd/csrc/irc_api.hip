@@ -1,0 +1,740 @@
+// irc_api.hip -- C ABI of the gfx950 density-peak library (see include/irc_hip.h).
+//
+// Host-side planning restates the reference's data preparation:
+//   prepare_page_site_ (matlab/irc2.m:3869-3910): site-local columns, drift per column
+//   search_knn_drift_  (matlab/irc2.m:3744-3748, 3768-3772): candidate runs of linked chunks
+// and replaces the per-(site, chunk) kernel launches by whole-problem launches.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "irc_common.h"
+#include "irc_kernels.h"
+#include "irc_screen.h"
+#include "../../include/irc_hip.h"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct irc_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    std::string err;
+    irc_stats stats{};
+    // device buffers, grown on demand and kept across calls
+    DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
+    DevBuf b_small, b_flags, b_knncol, b_knnt, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
+        b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
+    DevBuf b_scr[8];
+};
+
+namespace {
+
+irc_status fail(irc_ctx* c, irc_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return st;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, IRC_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+#define CHK(expr)                          \
+    do {                                   \
+        irc_status s_ = (expr);            \
+        if (s_ != IRC_OK) return s_;       \
+    } while (0)
+
+irc_status ensure(irc_ctx* ctx, DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (bytes <= b.cap) return IRC_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+        if (e != hipSuccess) return fail(ctx, IRC_E_HIP, "hipFree failed: %s", hipGetErrorString(e));
+    }
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        return fail(ctx, IRC_E_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    b.cap = want;
+    return IRC_OK;
+}
+
+template <class T>
+T* ptr(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+int pad_dim(int D) { return (D + 3) / 4 * 4; }
+
+irc_status check_opt(irc_ctx* ctx, const irc_options* opt) {
+    if (opt && opt->force_fail) return fail(ctx, IRC_E_FORCED, "failure forced by irc_options.force_fail");
+    return IRC_OK;
+}
+
+int persistent_grid(const irc_ctx* ctx, int waves_per_cu) { return ctx->n_cu * waves_per_cu; }
+
+// Link lists L(c) = {r : link[r + c*C] != 0}, ascending r.
+void build_links(int C, const uint8_t* link, std::vector<int>& lptr, std::vector<int>& lidx) {
+    lptr.assign((size_t)C + 1, 0);
+    lidx.clear();
+    for (int c = 0; c < C; ++c) {
+        for (int r = 0; r < C; ++r)
+            if (link[(size_t)r + (size_t)c * C]) lidx.push_back(r);
+        lptr[c + 1] = (int)lidx.size();
+    }
+}
+
+// Every chunk must be linked to itself (calc_drift_ always keeps a chunk among its own
+// nearest chunks, matlab/irc2.m:6433-6436), so every query has at least itself as candidate.
+bool links_have_diagonal(int C, const uint8_t* link) {
+    for (int c = 0; c < C; ++c)
+        if (!link[(size_t)c + (size_t)c * C]) return false;
+    return true;
+}
+
+// Host-side plan of one site problem (per-site API and the legacy A/B forms).
+struct HostPlan {
+    int D = 0, DP = 0, C = 0, k = 0;
+    int64_t n_cols = 0;
+    std::vector<float> feat;             // [n_cols][DP], chunk-grouped column order
+    std::vector<int> key, ret;           // per (permuted) column
+    std::vector<int> runb, runl;         // [C*2]
+    std::vector<int> lptr, lidx;
+    std::vector<int> qcol, qblk, qout;
+    std::vector<int64_t> inv;            // original column -> permuted column
+};
+
+// Upload a host plan into the context buffers and return the device view.
+irc_status upload_plan(irc_ctx* ctx, const HostPlan& h, DevPlan& p) {
+    const int64_t n = h.n_cols;
+    CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n * h.DP));
+    CHK(ensure(ctx, ctx->b_key, sizeof(int) * (size_t)n));
+    CHK(ensure(ctx, ctx->b_ret, sizeof(int) * (size_t)n));
+    CHK(ensure(ctx, ctx->b_runb, sizeof(int) * h.runb.size()));
+    CHK(ensure(ctx, ctx->b_runl, sizeof(int) * h.runl.size()));
+    CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * h.lptr.size()));
+    CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (h.lidx.size() + 1)));
+    CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * h.qcol.size()));
+    CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * h.qblk.size()));
+    CHK(ensure(ctx, ctx->b_qout, sizeof(int) * h.qout.size()));
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->b_feat.p, h.feat.data(), sizeof(float) * h.feat.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_key.p, h.key.data(), sizeof(int) * h.key.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_ret.p, h.ret.data(), sizeof(int) * h.ret.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_runb.p, h.runb.data(), sizeof(int) * h.runb.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_runl.p, h.runl.data(), sizeof(int) * h.runl.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, h.lptr.data(), sizeof(int) * h.lptr.size(), hipMemcpyHostToDevice, st));
+    if (!h.lidx.empty())
+        HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, h.lidx.data(), sizeof(int) * h.lidx.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_qcol.p, h.qcol.data(), sizeof(int) * h.qcol.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_qblk.p, h.qblk.data(), sizeof(int) * h.qblk.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_qout.p, h.qout.data(), sizeof(int) * h.qout.size(), hipMemcpyHostToDevice, st));
+    p.feat = ptr<float>(ctx->b_feat);
+    p.DP = h.DP;
+    p.D = h.D;
+    p.C = h.C;
+    p.k = h.k;
+    p.col_key = ptr<int>(ctx->b_key);
+    p.col_ret = ptr<int>(ctx->b_ret);
+    p.run_beg = ptr<int>(ctx->b_runb);
+    p.run_len = ptr<int>(ctx->b_runl);
+    p.link_ptr = ptr<int>(ctx->b_lptr);
+    p.link_idx = ptr<int>(ctx->b_lidx);
+    p.q_col = ptr<int>(ctx->b_qcol);
+    p.q_blk = ptr<int>(ctx->b_qblk);
+    p.q_out = ptr<int>(ctx->b_qout);
+    return IRC_OK;
+}
+
+// Plan of a per-site problem: columns regrouped by drift chunk (stable), one run per chunk.
+irc_status plan_site(irc_ctx* ctx, const irc_site_problem* pr, HostPlan& h) {
+    if (!pr || !pr->F || !pr->drift_of_col || !pr->link) return fail(ctx, IRC_E_ARG, "null pointer in irc_site_problem");
+    if (pr->D <= 0 || pr->k <= 0 || pr->n_cols <= 0 || pr->n_drift <= 0)
+        return fail(ctx, IRC_E_ARG, "invalid sizes (D=%d k=%d n_cols=%lld n_drift=%d)", pr->D, pr->k,
+                    (long long)pr->n_cols, pr->n_drift);
+    if (pr->D > IRC_DMAX) return fail(ctx, IRC_E_UNSUPPORTED, "D=%d exceeds %d", pr->D, IRC_DMAX);
+    if (pr->k > IRC_KMAX) return fail(ctx, IRC_E_UNSUPPORTED, "k=%d exceeds %d", pr->k, IRC_KMAX);
+    if (pr->n_cols >= (int64_t)1 << 31) return fail(ctx, IRC_E_UNSUPPORTED, "n_cols too large");
+    if (pr->in_begin < 0 || pr->n_in < 0 || pr->in_begin + pr->n_in > pr->n_cols)
+        return fail(ctx, IRC_E_ARG, "query range [%lld, +%lld) outside %lld columns", (long long)pr->in_begin,
+                    (long long)pr->n_in, (long long)pr->n_cols);
+    const int C = pr->n_drift;
+    const int64_t n = pr->n_cols;
+    if (!links_have_diagonal(C, pr->link)) return fail(ctx, IRC_E_ARG, "link(c,c) must be set for every chunk");
+    std::vector<int64_t> cnt((size_t)C + 1, 0);
+    for (int64_t j = 0; j < n; ++j) {
+        const int r = pr->drift_of_col[j];
+        if (r < 1 || r > C) return fail(ctx, IRC_E_ARG, "drift_of_col[%lld]=%d outside 1..%d", (long long)j, r, C);
+        cnt[r]++;
+    }
+    for (int c = 0; c < C; ++c) cnt[c + 1] += cnt[c];
+    h.D = pr->D;
+    h.DP = pad_dim(pr->D);
+    h.C = C;
+    h.k = pr->k;
+    h.n_cols = n;
+    h.feat.assign((size_t)n * h.DP, 0.0f);
+    h.key.resize(n);
+    h.ret.resize(n);
+    h.inv.resize(n);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t j = 0; j < n; ++j) {
+        const int r = pr->drift_of_col[j] - 1;
+        const int64_t dst = pos[r]++;
+        h.inv[j] = dst;
+        memcpy(&h.feat[(size_t)dst * h.DP], pr->F + (size_t)j * pr->D, sizeof(float) * pr->D);
+        int64_t key = pr->id_of_col ? pr->id_of_col[j] : j;
+        if (key < INT32_MIN || key > INT32_MAX) return fail(ctx, IRC_E_ARG, "id_of_col out of int32 range");
+        h.key[dst] = (int)key;
+        h.ret[dst] = (int)(j + 1);
+    }
+    h.runb.assign((size_t)C * 2, 0);
+    h.runl.assign((size_t)C * 2, 0);
+    for (int c = 0; c < C; ++c) {
+        h.runb[2 * c] = (int)cnt[c];
+        h.runl[2 * c] = (int)(cnt[c + 1] - cnt[c]);
+        h.runb[2 * c + 1] = (int)cnt[c + 1];
+    }
+    build_links(C, pr->link, h.lptr, h.lidx);
+    h.qcol.resize(pr->n_in);
+    h.qblk.resize(pr->n_in);
+    h.qout.resize(pr->n_in);
+    for (int64_t i = 0; i < pr->n_in; ++i) {
+        const int64_t j = pr->in_begin + i;
+        h.qcol[i] = (int)h.inv[j];
+        h.qblk[i] = pr->drift_of_col[j] - 1;
+        h.qout[i] = (int)i;
+    }
+    return IRC_OK;
+}
+
+// Plan of an A/B problem: chunk 0 = B (candidates), chunk 1 = A (queries), link 0 -> 1.
+void plan_ab(int D, int64_t nB, const float* B, int64_t nA, const float* A, int k, HostPlan& h) {
+    h.D = D;
+    h.DP = pad_dim(D);
+    h.C = 2;
+    h.k = k;
+    h.n_cols = nB + nA;
+    h.feat.assign((size_t)h.n_cols * h.DP, 0.0f);
+    h.key.resize(h.n_cols);
+    h.ret.resize(h.n_cols);
+    for (int64_t j = 0; j < nB; ++j) {
+        memcpy(&h.feat[(size_t)j * h.DP], B + (size_t)j * D, sizeof(float) * D);
+        h.key[j] = (int)j;
+        h.ret[j] = (int)(j + 1);
+    }
+    for (int64_t i = 0; i < nA; ++i) {
+        memcpy(&h.feat[(size_t)(nB + i) * h.DP], A + (size_t)i * D, sizeof(float) * D);
+        h.key[nB + i] = (int)(nB + i);
+        h.ret[nB + i] = 0;
+    }
+    h.runb = {0, (int)nB, (int)nB, (int)(nB + nA)};
+    h.runl = {(int)nB, 0, (int)nA, 0};
+    h.lptr = {0, 0, 1};
+    h.lidx = {0};
+    h.qcol.resize(nA);
+    h.qblk.assign(nA, 1);
+    h.qout.resize(nA);
+    for (int64_t i = 0; i < nA; ++i) {
+        h.qcol[i] = (int)(nB + i);
+        h.qout[i] = (int)i;
+    }
+}
+
+double event_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return ms;
+}
+
+int resolve_precision(const irc_options* opt) {
+    int prec = opt ? opt->precision : IRC_PREC_AUTO;
+    if (prec == IRC_PREC_AUTO) prec = IRC_PREC_BF16_SCREEN;
+    return prec;
+}
+
+// kNN pass over a device plan: exact path or screen + certification + exact fallback.
+irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int* knn_col, double* knn_t,
+                   double* dk2) {
+    hipStream_t st = ctx->stream;
+    ctx->stats.n_screen_fallback = 0;
+    if (prec == IRC_PREC_EXACT || !irc::screen_supported(p)) {
+        ctx->stats.precision_used = IRC_PREC_EXACT;
+        HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2, nullptr));
+        return IRC_OK;
+    }
+    ctx->stats.precision_used = prec;
+    irc::ScreenBufs sb;
+    CHK(ensure(ctx, ctx->b_scr[0], irc::screen_pack_bytes(p, ctx->stats.n_queries, prec)));
+    CHK(ensure(ctx, ctx->b_scr[1], sizeof(int) * (size_t)nq + 64));
+    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)p.C * 0 + irc::screen_aux_bytes(p)));
+    sb.pack = ctx->b_scr[0].p;
+    sb.fallback = ptr<int>(ctx->b_scr[1]);
+    sb.n_fallback = ptr<int>(ctx->b_scr[1]) + nq;  // counter after the list
+    sb.aux = ctx->b_scr[2].p;
+    HIPCHK(hipMemsetAsync(sb.n_fallback, 0, sizeof(int), st));
+    HIPCHK(irc::launch_screen_knn(st, p, nq, prec, k_over, sb, ctx->n_cu, knn_col, knn_t, dk2));
+    // exact kNN for queries whose screened candidate set could not be certified
+    HIPCHK(irc::launch_knn_exact_dev(st, p, sb.fallback, sb.n_fallback, nq, persistent_grid(ctx, 8), knn_col,
+                                     knn_t, dk2));
+    int nf = 0;
+    HIPCHK(hipMemcpyAsync(&nf, sb.n_fallback, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats.n_screen_fallback = nf;
+    return IRC_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+void irc_options_default(irc_options* opt) {
+    if (!opt) return;
+    memset(opt, 0, sizeof(*opt));
+    opt->precision = IRC_PREC_AUTO;
+    opt->k_over = 32;
+    opt->mem = IRC_MEM_HOST;
+}
+
+const char* irc_version(void) { return "irc_hip 1.0 gfx950 (density-peak hot path)"; }
+
+irc_status irc_ctx_create(irc_ctx** out, int device) {
+    if (!out) return IRC_E_ARG;
+    *out = nullptr;
+    irc_ctx* ctx = new (std::nothrow) irc_ctx();
+    if (!ctx) return IRC_E_NOMEM;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+        delete ctx;
+        return IRC_E_HIP;
+    }
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return IRC_E_HIP;
+    }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        ctx->n_cu = ncu;
+    for (auto& ev : ctx->ev) {
+        if (hipEventCreate(&ev) != hipSuccess) {
+            irc_ctx_destroy(ctx);
+            return IRC_E_HIP;
+        }
+    }
+    *out = ctx;
+    return IRC_OK;
+}
+
+void irc_ctx_destroy(irc_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf* all[] = {&ctx->b_F, &ctx->b_colspk, &ctx->b_feat, &ctx->b_key, &ctx->b_ret, &ctx->b_runb,
+                     &ctx->b_runl, &ctx->b_lptr, &ctx->b_lidx, &ctx->b_qcol, &ctx->b_qblk, &ctx->b_qout,
+                     &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_knnt, &ctx->b_dk2, &ctx->b_rhoslot,
+                     &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
+                     &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand};
+    for (DevBuf* b : all)
+        if (b->p) (void)hipFree(b->p);
+    for (auto& b : ctx->b_scr)
+        if (b.p) (void)hipFree(b.p);
+    for (auto& ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* irc_last_error(const irc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st) {
+    if (!ctx || !st) return IRC_E_ARG;
+    *st = ctx->stats;
+    return IRC_OK;
+}
+
+// ---------------------------------------------------------------------------
+irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_options* opt, float* dk_out,
+                         int32_t* knn_out) {
+    if (!ctx) return IRC_E_ARG;
+    CHK(check_opt(ctx, opt));
+    if (!dk_out || !knn_out) return fail(ctx, IRC_E_ARG, "null output");
+    HIPCHK(hipSetDevice(ctx->device));
+    HostPlan h;
+    CHK(plan_site(ctx, prob, h));
+    const int nq = (int)prob->n_in;
+    if (nq == 0) return IRC_OK;
+    DevPlan p;
+    CHK(upload_plan(ctx, h, p));
+    const int k = p.k;
+    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
+    hipStream_t st = ctx->stream;
+    ctx->stats = irc_stats{};
+    ctx->stats.n_queries = nq;
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, ptr<int>(ctx->b_knncol),
+                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
+                                  ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    HIPCHK(hipMemcpyAsync(dk_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(knn_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats.ms_knn = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
+    return IRC_OK;
+}
+
+irc_status irc_delta_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_options* opt,
+                           const float* rho_cols, float* dmin_out, int32_t* nneigh_out) {
+    if (!ctx) return IRC_E_ARG;
+    CHK(check_opt(ctx, opt));
+    if (!rho_cols || !dmin_out || !nneigh_out) return fail(ctx, IRC_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(ctx->device));
+    HostPlan h;
+    CHK(plan_site(ctx, prob, h));
+    const int nq = (int)prob->n_in;
+    if (nq == 0) return IRC_OK;
+    DevPlan p;
+    CHK(upload_plan(ctx, h, p));
+    std::vector<float> rho_perm((size_t)h.n_cols);
+    for (int64_t j = 0; j < h.n_cols; ++j) rho_perm[h.inv[j]] = rho_cols[j];
+    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)h.n_cols));
+    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)nq));
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho_perm.data(), sizeof(float) * h.n_cols, hipMemcpyHostToDevice, st));
+    ctx->stats = irc_stats{};
+    ctx->stats.n_queries = nq;
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->b_rhocol),
+                                   ptr<float>(ctx->b_out_delta), nullptr, ptr<int>(ctx->b_out_nn32)));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    HIPCHK(hipMemcpyAsync(dmin_out, ctx->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats.ms_delta = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.n_delta_full = nq;
+    return IRC_OK;
+}
+
+// ---------------------------------------------------------------------------
+irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_options* opt, irc_outputs* out) {
+    if (!ctx) return IRC_E_ARG;
+    CHK(check_opt(ctx, opt));
+    if (!b || !out || !b->F || !b->col_spike || !b->site_ptr || !b->site_n_primary || !b->lim_drift || !b->link)
+        return fail(ctx, IRC_E_ARG, "null pointer in irc_batch");
+    if (!out->rho || !out->delta || !out->nneigh) return fail(ctx, IRC_E_ARG, "null output (rho, delta, nneigh)");
+    const int S = b->n_sites, C = b->n_drift, D = b->D, k = b->k;
+    const int64_t N = b->n_spikes;
+    if (S <= 0 || C <= 0 || D <= 0 || k <= 0 || N <= 0) return fail(ctx, IRC_E_ARG, "invalid sizes");
+    if (D > IRC_DMAX) return fail(ctx, IRC_E_UNSUPPORTED, "D=%d exceeds %d", D, IRC_DMAX);
+    if (k > IRC_KMAX) return fail(ctx, IRC_E_UNSUPPORTED, "k=%d exceeds %d", k, IRC_KMAX);
+    if (N >= ((int64_t)1 << 31) - 1) return fail(ctx, IRC_E_UNSUPPORTED, "n_spikes must be < 2^31");
+    const bool dev = opt && opt->mem == IRC_MEM_DEVICE;
+    // host-side checks of the small descriptors
+    if (b->site_ptr[0] != 0) return fail(ctx, IRC_E_ARG, "site_ptr[0] must be 0");
+    std::vector<int64_t> prim((size_t)S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        if (b->site_ptr[s + 1] < b->site_ptr[s]) return fail(ctx, IRC_E_ARG, "site_ptr not monotone at %d", s);
+        if (b->site_n_primary[s] < 0 || b->site_n_primary[s] > b->site_ptr[s + 1] - b->site_ptr[s])
+            return fail(ctx, IRC_E_ARG, "site_n_primary[%d] invalid", s);
+        prim[s + 1] = prim[s] + b->site_n_primary[s];
+    }
+    const int64_t n_cols = b->site_ptr[S];
+    if (b->n_cols != 0 && b->n_cols != n_cols) return fail(ctx, IRC_E_ARG, "n_cols does not match site_ptr[S]");
+    if (prim[S] != N) return fail(ctx, IRC_E_ARG, "sum of site_n_primary (%lld) != n_spikes (%lld)", (long long)prim[S], (long long)N);
+    if (n_cols >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    for (int c = 0; c < C; ++c)
+        if (b->lim_drift[c + 1] < b->lim_drift[c]) return fail(ctx, IRC_E_ARG, "lim_drift not monotone");
+    if (!links_have_diagonal(C, b->link)) return fail(ctx, IRC_E_ARG, "link(c,c) must be set for every chunk");
+    std::vector<int> lptr, lidx;
+    build_links(C, b->link, lptr, lidx);
+
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int DP = pad_dim(D);
+    // small descriptors in one buffer: site_ptr[S+1], site_np[S], prim[S+1], lim[C+1]
+    const size_t n_small = (size_t)(S + 1) + S + (S + 1) + (C + 1);
+    CHK(ensure(ctx, ctx->b_small, sizeof(int64_t) * n_small));
+    int64_t* d_site_ptr = ptr<int64_t>(ctx->b_small);
+    int64_t* d_site_np = d_site_ptr + (S + 1);
+    int64_t* d_prim = d_site_np + S;
+    int64_t* d_lim = d_prim + (S + 1);
+    {
+        std::vector<int64_t> smallv(n_small);
+        memcpy(&smallv[0], b->site_ptr, sizeof(int64_t) * (S + 1));
+        memcpy(&smallv[S + 1], b->site_n_primary, sizeof(int64_t) * S);
+        memcpy(&smallv[2 * S + 1], prim.data(), sizeof(int64_t) * (S + 1));
+        memcpy(&smallv[3 * S + 2], b->lim_drift, sizeof(int64_t) * (C + 1));
+        HIPCHK(hipMemcpyAsync(d_site_ptr, smallv.data(), sizeof(int64_t) * n_small, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * lptr.size()));
+    CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (lidx.size() + 1)));
+    HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, st));
+    if (!lidx.empty()) HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, st));
+
+    const float* dF = b->F;
+    const int* dspk = b->col_spike;
+    if (!dev) {
+        CHK(ensure(ctx, ctx->b_F, sizeof(float) * (size_t)n_cols * D));
+        CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
+        HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(ctx->b_colspk.p, b->col_spike, sizeof(int) * (size_t)n_cols, hipMemcpyHostToDevice, st));
+        dF = ptr<float>(ctx->b_F);
+        dspk = ptr<int>(ctx->b_colspk);
+    }
+    CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n_cols * DP));
+    CHK(ensure(ctx, ctx->b_runb, sizeof(int) * (size_t)S * C * 2));
+    CHK(ensure(ctx, ctx->b_runl, sizeof(int) * (size_t)S * C * 2));
+    CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * (size_t)N));
+    CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * (size_t)N));
+    CHK(ensure(ctx, ctx->b_qout, sizeof(int) * (size_t)N));
+    CHK(ensure(ctx, ctx->b_flags, sizeof(int) * 4));
+    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)N * k));
+    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)N * k));
+    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
+    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
+    CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
+    float *o_rho, *o_delta, *o_dk = nullptr;
+    int64_t *o_nn, *o_knn = nullptr;
+    if (dev) {
+        o_rho = out->rho;
+        o_delta = out->delta;
+        o_nn = out->nneigh;
+        o_knn = out->knn;
+        o_dk = out->dk;
+    } else {
+        CHK(ensure(ctx, ctx->b_rhoslot, sizeof(float) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_out_nn, sizeof(int64_t) * (size_t)N));
+        o_rho = ptr<float>(ctx->b_rhoslot);
+        o_delta = ptr<float>(ctx->b_out_delta);
+        o_nn = ptr<int64_t>(ctx->b_out_nn);
+        if (out->knn) {
+            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int64_t) * (size_t)N * k));
+            o_knn = ptr<int64_t>(ctx->b_out_knn);
+        }
+        if (out->dk) {
+            CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)N));
+            o_dk = ptr<float>(ctx->b_out_dk);
+        }
+    }
+    DevPlan p;
+    p.feat = ptr<float>(ctx->b_feat);
+    p.DP = DP;
+    p.D = D;
+    p.C = C;
+    p.k = k;
+    p.col_key = dspk;
+    p.col_ret = dspk;
+    p.run_beg = ptr<int>(ctx->b_runb);
+    p.run_len = ptr<int>(ctx->b_runl);
+    p.link_ptr = ptr<int>(ctx->b_lptr);
+    p.link_idx = ptr<int>(ctx->b_lidx);
+    p.q_col = ptr<int>(ctx->b_qcol);
+    p.q_blk = ptr<int>(ctx->b_qblk);
+    p.q_out = ptr<int>(ctx->b_qout);
+    int* d_flags = ptr<int>(ctx->b_flags);
+    int* d_nunres = ptr<int>(ctx->b_unres) + N;
+    const int nq = (int)N;
+
+    ctx->stats = irc_stats{};
+    ctx->stats.n_queries = N;
+    HIPCHK(hipMemsetAsync(d_flags, 0, sizeof(int) * 2, st));
+    HIPCHK(hipMemsetAsync(d_nunres, 0, sizeof(int), st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(irc::launch_pack_fp32(st, n_cols, D, DP, dF, ptr<float>(ctx->b_feat)));
+    HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
+                                  ptr<int>(ctx->b_runl)));
+    HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
+                                     ptr<int>(ctx->b_qcol), ptr<int>(ctx->b_qblk), ptr<int>(ctx->b_qout), d_flags));
+    int flags = 0;
+    HIPCHK(hipMemcpyAsync(&flags, d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
+    if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
+    if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, ptr<int>(ctx->b_knncol),
+                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk, o_knn,
+                                  nullptr));
+    HIPCHK(hipEventRecord(ctx->ev[2], st));
+    HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
+    HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
+                                      ptr<float>(ctx->b_rhocol), o_delta, o_nn, ptr<int>(ctx->b_unres), d_nunres));
+    HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
+                                   ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
+    HIPCHK(hipEventRecord(ctx->ev[3], st));
+    int nunres = 0;
+    HIPCHK(hipMemcpyAsync(&nunres, d_nunres, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(out->rho, o_rho, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out->delta, o_delta, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(out->nneigh, o_nn, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st));
+        if (out->knn) HIPCHK(hipMemcpyAsync(out->knn, o_knn, sizeof(int64_t) * (size_t)N * k, hipMemcpyDeviceToHost, st));
+        if (out->dk) HIPCHK(hipMemcpyAsync(out->dk, o_dk, sizeof(float) * N, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats.n_delta_full = nunres;
+    ctx->stats.ms_knn = event_ms(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.ms_delta = event_ms(ctx->ev[2], ctx->ev[3]);
+    ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[3]);
+    return IRC_OK;
+}
+
+// ---------------------------------------------------------------------------
+irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A, int32_t k,
+                      float* K_out, int32_t* I_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (D <= 0 || k <= 0 || nB <= 0 || nA < 0 || !B || (nA > 0 && (!A || !K_out)))
+        return fail(ctx, IRC_E_ARG, "invalid arguments");
+    if (D > IRC_DMAX || k > IRC_KMAX) return fail(ctx, IRC_E_UNSUPPORTED, "D or k too large");
+    if (nA + nB >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    if (nA == 0) return IRC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HostPlan h;
+    plan_ab(D, nB, B, nA, A, k, h);
+    DevPlan p;
+    CHK(upload_plan(ctx, h, p));
+    const int nq = (int)nA;
+    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
+    hipStream_t st = ctx->stream;
+    ctx->stats = irc_stats{};
+    ctx->stats.n_queries = nq;
+    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
+                ptr<double>(ctx->b_dk2)));
+    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
+                                  ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
+    HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    if (I_out) HIPCHK(hipMemcpyAsync(I_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
+irc_status irc_delta_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, const float* RB, int64_t nA,
+                            const float* A, const float* RA, float* D_out, int32_t* N_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (D <= 0 || nB <= 0 || nA < 0 || !B || !RB || (nA > 0 && (!A || !RA || !D_out || !N_out)))
+        return fail(ctx, IRC_E_ARG, "invalid arguments");
+    if (D > IRC_DMAX) return fail(ctx, IRC_E_UNSUPPORTED, "D too large");
+    if (nA + nB >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    if (nA == 0) return IRC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HostPlan h;
+    plan_ab(D, nB, B, nA, A, 1, h);
+    DevPlan p;
+    CHK(upload_plan(ctx, h, p));
+    const int nq = (int)nA;
+    std::vector<float> rho((size_t)(nB + nA));
+    memcpy(rho.data(), RB, sizeof(float) * nB);
+    memcpy(rho.data() + nB, RA, sizeof(float) * nA);
+    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * rho.size()));
+    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)nq));
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho.data(), sizeof(float) * rho.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->b_rhocol),
+                                   ptr<float>(ctx->b_out_delta), nullptr, ptr<int>(ctx->b_out_nn32)));
+    HIPCHK(hipMemcpyAsync(D_out, ctx->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(N_out, ctx->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
+irc_status irc_rho_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, const int32_t* miSite12,
+                            const float* fet1, const float* fet2, float dc2, float* rho_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (D <= 0 || n1 < 0 || n12 < n1 || !miSite12 || !fet1 || !fet2 || (n1 > 0 && !rho_out))
+        return fail(ctx, IRC_E_ARG, "invalid arguments");
+    if (n12 >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many spikes");
+    if (n1 == 0) return IRC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t fb = sizeof(float) * (size_t)n12 * D;
+    CHK(ensure(ctx, ctx->b_F, 2 * fb));
+    CHK(ensure(ctx, ctx->b_key, sizeof(int) * 2 * (size_t)n12));
+    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)n1));
+    float* d_f1 = ptr<float>(ctx->b_F);
+    float* d_f2 = d_f1 + (size_t)n12 * D;
+    HIPCHK(hipMemcpyAsync(d_f1, fet1, fb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_f2, fet2, fb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_rho_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->b_key), d_f1, d_f2, dc2,
+                              ptr<float>(ctx->b_out_delta), persistent_grid(ctx, 16)));
+    HIPCHK(hipMemcpyAsync(rho_out, ctx->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
+irc_status irc_delta_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, const int32_t* miSite12,
+                              const float* fet1, const float* fet2, const float* rho12, float dc2, float* delta_out,
+                              uint32_t* nneigh_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (D <= 0 || n1 < 0 || n12 < n1 || !miSite12 || !fet1 || !fet2 || !rho12 ||
+        (n1 > 0 && (!delta_out || !nneigh_out)))
+        return fail(ctx, IRC_E_ARG, "invalid arguments");
+    if (n12 >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many spikes");
+    if (n1 == 0) return IRC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const size_t fb = sizeof(float) * (size_t)n12 * D;
+    CHK(ensure(ctx, ctx->b_F, 2 * fb));
+    CHK(ensure(ctx, ctx->b_key, sizeof(int) * 2 * (size_t)n12));
+    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n12));
+    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)n1));
+    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)n1));
+    float* d_f1 = ptr<float>(ctx->b_F);
+    float* d_f2 = d_f1 + (size_t)n12 * D;
+    HIPCHK(hipMemcpyAsync(d_f1, fet1, fb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_f2, fet2, fb, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho12, sizeof(float) * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_delta_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->b_key), d_f1, d_f2, ptr<float>(ctx->b_rhocol),
+                                dc2, ptr<float>(ctx->b_out_delta), ptr<unsigned>(ctx->b_out_nn32),
+                                persistent_grid(ctx, 16)));
+    HIPCHK(hipMemcpyAsync(delta_out, ctx->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->b_out_nn32.p, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
+}  // extern "C"

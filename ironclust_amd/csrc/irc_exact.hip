@@ -1,0 +1,425 @@
+// irc_exact.hip -- exact (FP64) kernels of the density-peak hot path for gfx950,
+// plus the device-side planning kernels.
+//
+// Canonical distance: d2 = fma-chain over dimensions of (double(q_d) - double(x_d))^2.
+// Zero padding beyond D adds fma(0,0,acc) = acc exactly, so loops run over the
+// padded dimension DP.
+//
+// Reference semantics restated (paths relative to the ironclust repository):
+//   kNN over the linked drift chunks of a site   matlab/irc2.m:3653-3668 (knn_cpu_ 4244-4263)
+//   rho = 1/d_k                                   matlab/irc2.m:3669
+//   nearest denser neighbour                      matlab/irc2.m:3703-3725 (delta_cpu_ 3981-3994)
+#include "irc_common.h"
+#include "irc_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ __forceinline__ double dist2_lds(const double* __restrict__ sq, const float* __restrict__ x, int DP) {
+    double acc = 0.0;
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    for (int d4 = 0; d4 < DP / 4; ++d4) {
+        float4 v = x4[d4];
+        double t;
+        t = sq[4 * d4 + 0] - (double)v.x; acc = fma(t, t, acc);
+        t = sq[4 * d4 + 1] - (double)v.y; acc = fma(t, t, acc);
+        t = sq[4 * d4 + 2] - (double)v.z; acc = fma(t, t, acc);
+        t = sq[4 * d4 + 3] - (double)v.w; acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+// wave-wide argmin of (t, key) carrying col; ties of identical (t,key) go to the lower lane
+__device__ __forceinline__ void wave_argmin(double& t, int& key, int& col, int& lane) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double ot = __shfl_xor(t, off, 64);
+        int ok = __shfl_xor(key, off, 64);
+        int oc = __shfl_xor(col, off, 64);
+        int ol = __shfl_xor(lane, off, 64);
+        bool take = irc_less(ot, ok, t, key) || (ot == t && ok == key && ol < lane);
+        if (take) { t = ot; key = ok; col = oc; lane = ol; }
+    }
+}
+
+// (t, key) < (t2, key of column c2); the column's key is read only on an exact tie
+__device__ __forceinline__ bool less_tc(const DevPlan& p, double t, int key, double t2, int c2) {
+    return t < t2 || (t == t2 && key < p.col_key[c2]);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exact kNN: one wavefront per query.  Lanes stride over the candidate columns of
+// the linked chunks, each lane keeps its own sorted list of up to k entries in LDS,
+// then k rounds of a wave-wide argmin merge the 64 lists.
+// Writes knn_col / knn_t (query-major, k per query) and dk2 per query.
+__global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restrict__ qlist, int nq_host,
+                                                  const int* __restrict__ nq_dev,
+                                                  int* __restrict__ knn_col, double* __restrict__ knn_t,
+                                                  double* __restrict__ dk2, int* __restrict__ n_cand) {
+    extern __shared__ double smem[];
+    double* sq = smem;                                    // [IRC_DMAX]
+    double* lt = smem + IRC_DMAX;                         // [k][64] distances
+    int* lc = reinterpret_cast<int*>(lt + p.k * 64);      // [k][64] columns (key = col_key[col])
+    const int lane = threadIdx.x;
+    const int k = p.k;
+    const int nq = nq_dev ? *nq_dev : nq_host;
+    for (int qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const int q = qlist ? qlist[qi] : qi;
+        const int qc = p.q_col[q];
+        const int blk = p.q_blk[q];
+        const int s = blk / p.C, c = blk - s * p.C;
+        __syncthreads();
+        if (lane < p.DP) sq[lane] = (double)p.feat[(int64_t)qc * p.DP + lane];
+        __syncthreads();
+        int n = 0;
+        int total = 0;
+        for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
+            const int b2 = s * p.C + p.link_idx[li];
+            for (int part = 0; part < 2; ++part) {
+                const int beg = p.run_beg[2 * b2 + part];
+                const int len = p.run_len[2 * b2 + part];
+                total += len;
+                for (int j = lane; j < len; j += 64) {
+                    const int col = beg + j;
+                    const double t = dist2_lds(sq, p.feat + (int64_t)col * p.DP, p.DP);
+                    const int key = p.col_key[col];
+                    int pos;
+                    if (n < k) pos = n++;
+                    else if (less_tc(p, t, key, lt[(k - 1) * 64 + lane], lc[(k - 1) * 64 + lane])) pos = k - 1;
+                    else continue;
+                    while (pos > 0 && less_tc(p, t, key, lt[(pos - 1) * 64 + lane], lc[(pos - 1) * 64 + lane])) {
+                        lt[pos * 64 + lane] = lt[(pos - 1) * 64 + lane];
+                        lc[pos * 64 + lane] = lc[(pos - 1) * 64 + lane];
+                        --pos;
+                    }
+                    lt[pos * 64 + lane] = t;
+                    lc[pos * 64 + lane] = col;
+                }
+            }
+        }
+        const int kk = total < k ? total : k;
+        int head = 0;
+        double last_t = 0.0;
+        int last_c = -1;
+        for (int j = 0; j < kk; ++j) {
+            double t = head < n ? lt[head * 64 + lane] : __builtin_inf();
+            int col = head < n ? lc[head * 64 + lane] : -1;
+            int key = head < n ? p.col_key[col] : 0x7fffffff;
+            int wl = lane;
+            wave_argmin(t, key, col, wl);
+            if (wl == lane) ++head;
+            if (lane == 0) {
+                knn_col[(int64_t)q * k + j] = col;
+                knn_t[(int64_t)q * k + j] = t;
+            }
+            last_t = t;
+            last_c = col;
+        }
+        if (lane == 0) {
+            for (int j = kk; j < k; ++j) {
+                knn_col[(int64_t)q * k + j] = last_c;
+                knn_t[(int64_t)q * k + j] = last_t;
+            }
+            dk2[q] = last_t;
+            if (n_cand) n_cand[q] = total;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Finish the kNN pass: rho (batch mode) or d_k (site mode) and the reported ids.
+//   mode 0 (batch): rho_slot[slot] = float(1/sqrt(dk2)); dk_slot (opt) = float(sqrt(dk2));
+//                   knn64[slot*k+j] = col_ret (opt)
+//   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret
+__global__ void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
+                             const double* __restrict__ dk2, float* __restrict__ rho_slot,
+                             float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
+                             int* __restrict__ knn32) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nq) {
+        const int slot = p.q_out[i];
+        const double d = dk2[i];
+        if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
+        if (dk_slot) dk_slot[slot] = (float)sqrt(d);
+    }
+    const int64_t total = (int64_t)nq * p.k;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t qi = e / p.k;
+        const int j = (int)(e - qi * p.k);
+        const int slot = p.q_out[qi];
+        const int v = p.col_ret[knn_col[e]];
+        if (knn64) knn64[(int64_t)slot * p.k + j] = v;
+        if (knn32) knn32[(int64_t)slot * p.k + j] = v;
+    }
+}
+
+// rho per column from rho per slot (batch mode: slot = spike id - 1 = col_key - 1)
+__global__ void k_rho_cols(int64_t n_cols, const int* __restrict__ col_key, const float* __restrict__ rho_slot,
+                           float* __restrict__ rho_col) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_cols) rho_col[i] = rho_slot[col_key[i] - 1];
+}
+
+// ---------------------------------------------------------------------------
+// Delta from the kNN list: the first kNN entry (in (d2,id) order) whose rho exceeds
+// the query's is the nearest denser spike, because kNN(a) is a prefix of C(a) in key
+// order.  Queries without such an entry are appended to `unres`.
+__global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_col,
+                                 const double* __restrict__ knn_t, const float* __restrict__ rho_col,
+                                 float* __restrict__ delta_slot, int64_t* __restrict__ nneigh_slot,
+                                 int* __restrict__ unres, int* __restrict__ n_unres) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const int qc = p.q_col[q];
+    const float ra = rho_col[qc];
+    const int k = p.k;
+    for (int j = 0; j < k; ++j) {
+        const int col = knn_col[(int64_t)q * k + j];
+        if (rho_col[col] > ra) {
+            const int slot = p.q_out[q];
+            const float dl = (float)(sqrt(knn_t[(int64_t)q * k + j]) * (double)ra);
+            if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
+                delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
+                nneigh_slot[slot] = p.col_ret[qc];
+            } else {
+                delta_slot[slot] = dl;
+                nneigh_slot[slot] = p.col_ret[col];
+            }
+            return;
+        }
+    }
+    const int u = atomicAdd(n_unres, 1);
+    unres[u] = q;
+}
+
+// ---------------------------------------------------------------------------
+// Exact nearest denser neighbour: one wavefront per query, lanes over candidates.
+//   mode 0 (batch): delta = float(sqrt(d2)*rho_a) with the sentinel rule; nneigh int64
+//   mode 1 (site) : dmin = float(sqrt(d2)) or +Inf; nneigh32 = col_ret (own when none)
+// The query list is qlist[0 .. *nq_dev) when nq_dev is set, else qlist[0..nq) / identity.
+__global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __restrict__ qlist, int nq,
+                                                    const int* __restrict__ nq_dev, int mode,
+                                                    const float* __restrict__ rho_col,
+                                                    float* __restrict__ delta_slot,
+                                                    int64_t* __restrict__ nneigh64,
+                                                    int* __restrict__ nneigh32) {
+    __shared__ double sq[IRC_DMAX];
+    const int lane = threadIdx.x;
+    const int n = nq_dev ? *nq_dev : nq;
+    for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
+        const int q = qlist ? qlist[qi] : qi;
+        const int qc = p.q_col[q];
+        const int blk = p.q_blk[q];
+        const int s = blk / p.C, c = blk - s * p.C;
+        const float ra = rho_col[qc];
+        __syncthreads();
+        if (lane < p.DP) sq[lane] = (double)p.feat[(int64_t)qc * p.DP + lane];
+        __syncthreads();
+        double bt = __builtin_inf();
+        int bk = 0x7fffffff, bc = -1;
+        for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
+            const int b2 = s * p.C + p.link_idx[li];
+            for (int part = 0; part < 2; ++part) {
+                const int beg = p.run_beg[2 * b2 + part];
+                const int len = p.run_len[2 * b2 + part];
+                for (int j = lane; j < len; j += 64) {
+                    const int col = beg + j;
+                    if (!(rho_col[col] > ra)) continue;
+                    const double t = dist2_lds(sq, p.feat + (int64_t)col * p.DP, p.DP);
+                    const int key = p.col_key[col];
+                    if (bc < 0 || irc_less(t, key, bt, bk)) { bt = t; bk = key; bc = col; }
+                }
+            }
+        }
+        int wl = lane;
+        if (bc < 0) { bt = __builtin_inf(); bk = 0x7fffffff; }
+        wave_argmin(bt, bk, bc, wl);
+        if (lane == 0) {
+            const int slot = p.q_out[q];
+            if (mode == 0) {
+                float dl = bc < 0 ? __builtin_inff() : (float)(sqrt(bt) * (double)ra);
+                if (bc < 0 || __builtin_isnan(dl) || __builtin_isinf(dl)) {
+                    delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
+                    nneigh64[slot] = p.col_ret[qc];
+                } else {
+                    delta_slot[slot] = dl;
+                    nneigh64[slot] = p.col_ret[bc];
+                }
+            } else {
+                if (bc < 0) {
+                    delta_slot[slot] = __builtin_inff();
+                    nneigh32[slot] = p.col_ret[qc];
+                } else {
+                    delta_slot[slot] = (float)sqrt(bt);
+                    nneigh32[slot] = p.col_ret[bc];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Planning kernels (batch form).
+
+// Pack D x n (column-major) features into [n][DP] rows, zero padded.
+__global__ void k_pack_fp32(int64_t n_cols, int D, int DP, const float* __restrict__ F, float* __restrict__ feat) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = n_cols * DP;
+    if (e >= total) return;
+    const int64_t col = e / DP;
+    const int d = (int)(e - col * DP);
+    feat[e] = d < D ? F[col * D + d] : 0.0f;
+}
+
+__device__ __forceinline__ int64_t lower_bound_i32(const int* a, int64_t lo, int64_t hi, int64_t v) {
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// One thread per (site, chunk, part): the column run of that block part.
+__global__ void k_block_runs(int S, int C, const int64_t* __restrict__ site_ptr, const int64_t* __restrict__ site_np,
+                             const int64_t* __restrict__ lim, const int* __restrict__ col_spike,
+                             int* __restrict__ run_beg, int* __restrict__ run_len) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)S * C * 2) return;
+    const int part = (int)(e & 1);
+    const int64_t b = e >> 1;
+    const int s = (int)(b / C), c = (int)(b - (int64_t)s * C);
+    const int64_t lo = part == 0 ? site_ptr[s] : site_ptr[s] + site_np[s];
+    const int64_t hi = part == 0 ? site_ptr[s] + site_np[s] : site_ptr[s + 1];
+    const int64_t a0 = lower_bound_i32(col_spike, lo, hi, lim[c]);
+    const int64_t a1 = lower_bound_i32(col_spike, a0, hi, lim[c + 1]);
+    run_beg[e] = (int)a0;
+    run_len[e] = (int)(a1 - a0);
+}
+
+// One thread per query (primary column): column, block and output slot.
+// Flags bit 0: spike id out of range, bit 1: id outside the drift limits,
+// bit 2: ids not ascending within a site part.
+__global__ void k_batch_queries(int64_t N, int S, int C, const int64_t* __restrict__ site_ptr,
+                                const int64_t* __restrict__ prim_ptr, const int64_t* __restrict__ lim,
+                                const int* __restrict__ col_spike, int* __restrict__ q_col,
+                                int* __restrict__ q_blk, int* __restrict__ q_out, int* __restrict__ flags) {
+    const int64_t qi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= N) return;
+    int lo = 0, hi = S;  // largest s with prim_ptr[s] <= qi
+    while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (prim_ptr[mid] <= qi) lo = mid; else hi = mid; }
+    const int s = lo;
+    const int64_t col = site_ptr[s] + (qi - prim_ptr[s]);
+    const int id = col_spike[col];
+    if (id < 1 || id > N) { atomicOr(flags, 1); return; }
+    int cl = 0, ch = C;  // largest c with lim[c] <= id
+    if (id < lim[0] || id >= lim[C]) { atomicOr(flags, 2); return; }
+    while (ch - cl > 1) { int mid = (cl + ch) >> 1; if (lim[mid] <= id) cl = mid; else ch = mid; }
+    if (qi > prim_ptr[s] && col_spike[col - 1] >= id) atomicOr(flags, 4);
+    q_col[qi] = (int)col;
+    q_blk[qi] = s * C + cl;
+    q_out[qi] = id - 1;
+}
+
+// Secondary-part ordering and range check.
+__global__ void k_check_secondary(int64_t N, int S, const int64_t* __restrict__ site_ptr,
+                                  const int64_t* __restrict__ site_np, const int* __restrict__ col_spike,
+                                  int* __restrict__ flags) {
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= site_ptr[S]) return;
+    int lo = 0, hi = S;
+    while (hi - lo > 1) { int mid = (lo + hi) >> 1; if (site_ptr[mid] <= col) lo = mid; else hi = mid; }
+    const int64_t b = site_ptr[lo] + site_np[lo];
+    if (col < b) return;
+    const int id = col_spike[col];
+    if (id < 1 || id > N) atomicOr(flags, 1);
+    if (col > b && col_spike[col - 1] >= id) atomicOr(flags, 4);
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+namespace irc {
+
+hipError_t launch_pack_fp32(hipStream_t st, int64_t n_cols, int D, int DP, const float* F, float* feat) {
+    const int64_t total = n_cols * DP;
+    if (total == 0) return hipSuccess;
+    k_pack_fp32<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(n_cols, D, DP, F, feat);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_runs(hipStream_t st, int S, int C, const int64_t* site_ptr, const int64_t* site_np,
+                             const int64_t* lim, const int* col_spike, int* run_beg, int* run_len) {
+    const int64_t total = (int64_t)S * C * 2;
+    k_block_runs<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(S, C, site_ptr, site_np, lim, col_spike, run_beg, run_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_queries(hipStream_t st, int64_t N, int S, int C, const int64_t* site_ptr,
+                                const int64_t* site_np, const int64_t* prim_ptr, const int64_t* lim,
+                                const int* col_spike, int64_t n_cols, int* q_col, int* q_blk, int* q_out, int* flags) {
+    k_batch_queries<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(N, S, C, site_ptr, prim_ptr, lim, col_spike,
+                                                                 q_col, q_blk, q_out, flags);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (n_cols > 0)
+        k_check_secondary<<<(unsigned)((n_cols + 255) / 256), 256, 0, st>>>(N, S, site_ptr, site_np, col_spike, flags);
+    return hipGetLastError();
+}
+
+size_t knn_exact_lds_bytes(int k) { return sizeof(double) * IRC_DMAX + (size_t)k * 64 * (8 + 4); }
+
+hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, int grid,
+                            int* knn_col, double* knn_t, double* dk2, int* n_cand) {
+    if (nq <= 0) return hipSuccess;
+    const int g = grid < nq ? grid : nq;
+    k_knn_exact<<<g, 64, knn_exact_lds_bytes(p.k), st>>>(p, qlist, nq, nullptr, knn_col, knn_t, dk2, n_cand);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,
+                                int grid, int* knn_col, double* knn_t, double* dk2) {
+    if (nq_max <= 0) return hipSuccess;
+    const int g = grid < nq_max ? grid : nq_max;
+    k_knn_exact<<<g, 64, knn_exact_lds_bytes(p.k), st>>>(p, qlist, nq_max, nq_dev, knn_col, knn_t, dk2, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
+                             const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32) {
+    if (nq <= 0) return hipSuccess;
+    const int64_t total = (int64_t)nq * p.k;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks < (nq + 255) / 256) blocks = (nq + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < (nq + 255) / 256) blocks = (nq + 255) / 256;
+    k_knn_finish<<<(unsigned)blocks, 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot, knn64, knn32);
+    return hipGetLastError();
+}
+
+hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, const float* rho_slot, float* rho_col) {
+    if (n_cols <= 0) return hipSuccess;
+    k_rho_cols<<<(unsigned)((n_cols + 255) / 256), 256, 0, st>>>(n_cols, col_key, rho_slot, rho_col);
+    return hipGetLastError();
+}
+
+hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, const double* knn_t,
+                                 const float* rho_col, float* delta_slot, int64_t* nneigh_slot, int* unres,
+                                 int* n_unres) {
+    if (nq <= 0) return hipSuccess;
+    k_delta_from_knn<<<(nq + 255) / 256, 256, 0, st>>>(p, nq, knn_col, knn_t, rho_col, delta_slot, nneigh_slot,
+                                                       unres, n_unres);
+    return hipGetLastError();
+}
+
+hipError_t launch_delta_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, const int* nq_dev,
+                              int grid, int mode, const float* rho_col, float* delta_slot, int64_t* nneigh64,
+                              int* nneigh32) {
+    if (nq <= 0) return hipSuccess;
+    const int g = grid < nq ? grid : nq;
+    k_delta_exact<<<g, 64, 0, st>>>(p, qlist, nq, nq_dev, mode, rho_col, delta_slot, nneigh64, nneigh32);
+    return hipGetLastError();
+}
+
+}  // namespace irc

@@ -1,0 +1,177 @@
+"""Host-side mirror of irc2's hot-path functions, calling the gfx950 library.
+
+Names, argument meaning and orientation follow the reference (matlab/irc2.m) so a
+caller can switch call sites one for one.  Arrays use MATLAB orientation: features
+are D x n (one column per spike), indices returned are 1-based.
+
+  search_knn_drift_   matlab/irc2.m:3731-3789
+  search_delta_drift_ matlab/irc2.m:3794-3865
+  cuda_knn_           matlab/irc2.m:4202-4238
+  sort_page_          matlab/irc2.m:3456-3503 (rho_page_, delta_page_, rho_paged_site_,
+                      delta_paged_site_ folded into one batched call)
+  plan_sort_page_     matlab/irc2.m:3914-3946
+  prepare_page_site_  matlab/irc2.m:3869-3910 (site_layout builds the same column grouping
+                      for every site at once)
+
+There is no CPU fallback: a library error raises IrcError (the reference's callers
+catch errors and fall back to their own CPU code, matlab/irc2.m:3643-3652).
+"""
+import numpy as np
+
+from . import libirc
+
+_CTX = {}
+
+
+def get_ctx(device=0):
+    """One context per device per process, like the reference's `persistent CK`."""
+    ctx = _CTX.get(device)
+    if ctx is None:
+        ctx = libirc.Context(device)
+        _CTX[device] = ctx
+    return ctx
+
+
+def _opts(P):
+    P = P or {}
+    prec = P.get("precision", libirc.PREC_AUTO)
+    if isinstance(prec, str):
+        prec = {"exact": libirc.PREC_EXACT, "fp32": libirc.PREC_FP32, "bf16": libirc.PREC_BF16_SCREEN,
+                "auto": libirc.PREC_AUTO}[prec]
+    return libirc.make_options(precision=prec, k_over=int(P.get("k_over", 32)),
+                               force_fail=bool(P.get("force_fail", False)), flags=int(P.get("flags", 0)))
+
+
+def search_knn_drift_(vl_in, mrFet_out, viDrift_out, mlDrift1, P=None, viSpk_out=None):
+    """[vrKnn_in, miKnn_in] = search_knn_drift_(vl_in, mrFet_out, viDrift_out, mlDrift1, P).
+    vl_in: logical over columns (a contiguous run); mrFet_out: D x nF single;
+    viDrift_out: 1-based chunk per column; mlDrift1: C x C logical.
+    Returns vrKnn_in (n_in,) single = distance to the k-th neighbour and miKnn_in (knn x n_in)
+    int32 1-based column indices.  viSpk_out (optional) gives spike ids for tie-breaking."""
+    P = P or {}
+    knn = int(P.get("knn", 30))
+    vl_in = np.asarray(vl_in, dtype=bool).ravel()
+    idx = np.flatnonzero(vl_in)
+    if idx.size == 0:
+        return np.zeros(0, np.float32), np.zeros((knn, 0), np.int32)
+    if idx[-1] - idx[0] + 1 != idx.size:
+        raise ValueError("vl_in must mark a contiguous run of columns")
+    F = np.ascontiguousarray(np.asarray(mrFet_out, dtype=np.float32).T)
+    dk, I = get_ctx(P.get("device", 0)).knn_drift(F, viDrift_out, np.asarray(mlDrift1, dtype=np.uint8),
+                                                  idx[0], idx.size, knn, id_of_col=viSpk_out, opts=_opts(P))
+    return dk, np.ascontiguousarray(I.T)
+
+
+def search_delta_drift_(vl_in, mrFet_out, vrRho_out, viDrift_out, mlDrift1, P=None, viSpk_out=None):
+    """[vrDelta_in, viNneigh_in] = search_delta_drift_(vl_in, mrFet_out, vrRho_out, viDrift_out, mlDrift1, P).
+    vrDelta_in is the distance (not yet multiplied by rho; +Inf when no denser column exists),
+    viNneigh_in the 1-based column (own column when none)."""
+    P = P or {}
+    vl_in = np.asarray(vl_in, dtype=bool).ravel()
+    idx = np.flatnonzero(vl_in)
+    if idx.size == 0:
+        return np.zeros(0, np.float32), np.zeros(0, np.int32)
+    if idx[-1] - idx[0] + 1 != idx.size:
+        raise ValueError("vl_in must mark a contiguous run of columns")
+    F = np.ascontiguousarray(np.asarray(mrFet_out, dtype=np.float32).T)
+    return get_ctx(P.get("device", 0)).delta_drift(F, viDrift_out, np.asarray(mlDrift1, dtype=np.uint8), idx[0],
+                                                   idx.size, vrRho_out, id_of_col=viSpk_out, opts=_opts(P))
+
+
+def cuda_knn_(mrFet2, mrFet1, P=None):
+    """[vrKnn, fGpu, miKnn] = cuda_knn_(mrFet2, mrFet1, P): kNN of the columns of mrFet1 among
+    the columns of mrFet2 (knn = min(knn, n2) rows, as the reference)."""
+    P = P or {}
+    knn = int(P.get("knn", 30))
+    B = np.ascontiguousarray(np.asarray(mrFet2, dtype=np.float32).T)
+    A = np.ascontiguousarray(np.asarray(mrFet1, dtype=np.float32).T)
+    knn = min(knn, B.shape[0])
+    K, I = get_ctx(P.get("device", 0)).knn_ab(B, A, knn)
+    return K, 1, np.ascontiguousarray(I.T)
+
+
+def site_layout(ds, F1=None):
+    """Group spike features by site (the _fet_/_fet2_ layout, matlab/irc2.m:4583-4618):
+    for each site, the spikes whose primary site it is (primary features, ascending id), then
+    the spikes whose secondary site it is (secondary features, ascending id).
+    Returns numpy/torch arrays for irc_rho_delta_batch."""
+    import torch
+    site1 = ds["site1"]
+    dev = site1.device
+    N = ds["n_spikes"]
+    S = ds["n_sites"]
+    ids = torch.arange(1, N + 1, device=dev, dtype=torch.int32)
+    key1 = (site1.long() - 1) * 2
+    F1 = ds["F1"] if F1 is None else F1
+    if ds.get("site2") is not None:
+        s2 = ds["site2"]
+        m = s2 > 0
+        key2 = (s2[m].long() - 1) * 2 + 1
+        keys = torch.cat([key1, key2])
+        allids = torch.cat([ids, ids[m]])
+        src = torch.cat([torch.zeros(N, dtype=torch.bool, device=dev), torch.ones(int(m.sum()), dtype=torch.bool, device=dev)])
+        rows = torch.cat([torch.arange(N, device=dev), torch.nonzero(m).flatten()])
+    else:
+        keys = key1
+        allids = ids
+        src = None
+        rows = torch.arange(N, device=dev)
+    order = torch.sort(keys, stable=True).indices
+    col_spike = allids[order].contiguous()
+    rows_o = rows[order]
+    if src is None:
+        F = F1[rows_o].contiguous()
+    else:
+        so = src[order]
+        F = torch.empty((order.numel(), F1.shape[1]), dtype=torch.float32, device=dev)
+        F[~so] = F1[rows_o[~so]]
+        F[so] = ds["F2"][rows_o[so]]
+    cnt = torch.bincount(keys[order], minlength=2 * S).reshape(S, 2).cpu().numpy().astype(np.int64)
+    site_ptr = np.concatenate([[0], np.cumsum(cnt.sum(1))]).astype(np.int64)
+    return dict(F=F, col_spike=col_spike, site_ptr=site_ptr, site_n_primary=cnt[:, 0].copy(),
+                lim_drift=np.asarray(ds["lim"], dtype=np.int64), link=np.asarray(ds["link"], dtype=np.uint8),
+                n_spikes=N, k=ds["k"])
+
+
+def sort_page_(layout, P=None, want_knn=True):
+    """[vrRho, vrDelta, viNneigh] (+ kNN, k x N) for every site and drift chunk in one call.
+    rho is unnormalised (sort_ divides by max afterwards, matlab/irc2.m:3268)."""
+    P = P or {}
+    ctx = get_ctx(P.get("device", 0))
+    host = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in layout.items()}
+    r = ctx.rho_delta_batch(host, int(P.get("knn", layout.get("k", 30))), opts=_opts(P), want_knn=want_knn)
+    out = [r["rho"], r["delta"], r["nneigh"]]
+    if want_knn:
+        out.append(np.ascontiguousarray(r["knn"].T))
+    return tuple(out)
+
+
+def plan_sort_page_(nTime_drift, viLim_drift, nTime_max_drift=256):
+    """Pages of drift chunks (matlab/irc2.m:3914-3946): returns (miSpk_lim_out, miSpk_lim_in,
+    miDrift_lim_out, miDrift_lim_in), 1-based inclusive limits, one row per page."""
+    n = int(nTime_drift)
+    width = int(nTime_max_drift)
+    lim = np.asarray(viLim_drift, dtype=np.int64)
+    quarter, half = width // 4, width // 2
+    n_pages = n // half
+    if n_pages == 0:
+        so = np.array([[lim[0], lim[-1] - 1]], np.int64)
+        do = np.array([[1, len(lim) - 1]], np.int32)
+        return so, so.copy(), do, do.copy()
+    so = np.zeros((n_pages, 2), np.int64)
+    si = np.zeros((n_pages, 2), np.int64)
+    do = np.zeros((n_pages, 2), np.int32)
+    di = np.zeros((n_pages, 2), np.int32)
+    ib_prev = 0
+    for p in range(1, n_pages + 1):
+        iB = min((p - 1) * half + width, n)
+        iA = max(1, iB - width + 1)
+        ia = 1 if p == 1 else min(iA + quarter, n)
+        ia = max(ib_prev + 1, ia)
+        ib = n if p == n_pages else max(iB - quarter, 1)
+        ib_prev = ib
+        do[p - 1] = (iA, iB)
+        di[p - 1] = (ia, ib)
+        si[p - 1] = (lim[ia - 1], lim[ib] - 1)
+        so[p - 1] = (lim[iA - 1], lim[iB] - 1)
+    return so, si, do, di

@@ -1,0 +1,310 @@
+"""ctypes binding of libirc_hip.so (the C ABI in include/irc_hip.h).
+
+This module only marshals arguments; every computation runs in the gfx950 library.
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libirc_hip.so")
+
+IRC_OK, IRC_E_ARG, IRC_E_NOMEM, IRC_E_HIP, IRC_E_UNSUPPORTED, IRC_E_FORCED = range(6)
+PREC_EXACT, PREC_FP32, PREC_BF16_SCREEN, PREC_AUTO = range(4)
+MEM_HOST, MEM_DEVICE = 0, 1
+
+EXPORTED = [
+    "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
+    "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
+    "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc",
+]
+
+_c_f32p = ctypes.POINTER(ctypes.c_float)
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_u32p = ctypes.POINTER(ctypes.c_uint32)
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("k_over", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("force_fail", ctypes.c_int32), ("mem", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class SiteProblem(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int32), ("k", ctypes.c_int32), ("n_cols", ctypes.c_int64),
+                ("F", _c_f32p), ("drift_of_col", _c_i32p), ("id_of_col", _c_i64p),
+                ("n_drift", ctypes.c_int32), ("link", _c_u8p),
+                ("in_begin", ctypes.c_int64), ("n_in", ctypes.c_int64)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("n_spikes", ctypes.c_int64), ("D", ctypes.c_int32), ("k", ctypes.c_int32),
+                ("n_sites", ctypes.c_int32), ("n_drift", ctypes.c_int32), ("n_cols", ctypes.c_int64),
+                ("F", _c_f32p), ("col_spike", _c_i32p), ("site_ptr", _c_i64p),
+                ("site_n_primary", _c_i64p), ("lim_drift", _c_i64p), ("link", _c_u8p)]
+
+
+class Outputs(ctypes.Structure):
+    _fields_ = [("rho", _c_f32p), ("delta", _c_f32p), ("nneigh", _c_i64p), ("knn", _c_i64p), ("dk", _c_f32p)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_queries", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
+                ("n_screen_fallback", ctypes.c_int64), ("n_delta_full", ctypes.c_int64),
+                ("ms_total", ctypes.c_double), ("ms_knn", ctypes.c_double), ("ms_delta", ctypes.c_double),
+                ("precision_used", ctypes.c_int32)]
+
+
+class IrcError(RuntimeError):
+    """A non-OK irc_status; `status` holds the code (the MEX maps it to irc:hip errors)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"irc_hip status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load libirc_hip.so; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with ironclust_amd.build.build_lib()")
+    lib = ctypes.CDLL(p)
+    lib.irc_options_default.argtypes = [ctypes.POINTER(Options)]
+    lib.irc_options_default.restype = None
+    lib.irc_ctx_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
+    lib.irc_ctx_create.restype = ctypes.c_int
+    lib.irc_ctx_destroy.argtypes = [ctypes.c_void_p]
+    lib.irc_ctx_destroy.restype = None
+    lib.irc_last_error.argtypes = [ctypes.c_void_p]
+    lib.irc_last_error.restype = ctypes.c_char_p
+    lib.irc_version.argtypes = []
+    lib.irc_version.restype = ctypes.c_char_p
+    lib.irc_knn_drift.argtypes = [ctypes.c_void_p, ctypes.POINTER(SiteProblem), ctypes.POINTER(Options),
+                                  _c_f32p, _c_i32p]
+    lib.irc_knn_drift.restype = ctypes.c_int
+    lib.irc_delta_drift.argtypes = [ctypes.c_void_p, ctypes.POINTER(SiteProblem), ctypes.POINTER(Options),
+                                    _c_f32p, _c_f32p, _c_i32p]
+    lib.irc_delta_drift.restype = ctypes.c_int
+    lib.irc_rho_delta_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
+                                        ctypes.POINTER(Outputs)]
+    lib.irc_rho_delta_batch.restype = ctypes.c_int
+    lib.irc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+    lib.irc_get_stats.restype = ctypes.c_int
+    lib.irc_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_f32p,
+                               ctypes.c_int32, _c_f32p, _c_i32p]
+    lib.irc_knn_ab.restype = ctypes.c_int
+    lib.irc_delta_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, _c_f32p,
+                                     ctypes.c_int64, _c_f32p, _c_f32p, _c_f32p, _c_i32p]
+    lib.irc_delta_knn_ab.restype = ctypes.c_int
+    lib.irc_rho_drift_dc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _c_i32p,
+                                     _c_f32p, _c_f32p, ctypes.c_float, _c_f32p]
+    lib.irc_rho_drift_dc.restype = ctypes.c_int
+    lib.irc_delta_drift_dc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _c_i32p,
+                                       _c_f32p, _c_f32p, _c_f32p, ctypes.c_float, _c_f32p, _c_u32p]
+    lib.irc_delta_drift_dc.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _p(a, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ct)
+
+
+def _dptr(t, ct):
+    """ctypes pointer of a device tensor (torch) or None."""
+    if t is None:
+        return None
+    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), ct)
+
+
+def make_options(precision=PREC_AUTO, k_over=32, force_fail=False, mem=MEM_HOST, flags=0):
+    lib = load()
+    o = Options()
+    lib.irc_options_default(ctypes.byref(o))
+    o.precision = precision
+    o.k_over = k_over
+    o.force_fail = 1 if force_fail else 0
+    o.mem = mem
+    o.flags = flags
+    return o
+
+
+class Context:
+    """One irc_ctx (device buffers, stream) -- the role of MATLAB's `persistent CK`."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        st = self.lib.irc_ctx_create(ctypes.byref(h), int(device))
+        if st != IRC_OK:
+            raise IrcError(st, f"irc_ctx_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.irc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != IRC_OK:
+            raise IrcError(st, self.lib.irc_last_error(self.h).decode(errors="replace"))
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.irc_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    # ---- per-site forms (search_knn_drift_ / search_delta_drift_) --------------
+    def _site_problem(self, F, drift_of_col, link, in_begin, n_in, k, id_of_col):
+        F = np.ascontiguousarray(F, dtype=np.float32)
+        n_cols, D = F.shape
+        drift = np.ascontiguousarray(drift_of_col, dtype=np.int32)
+        link = np.asfortranarray(np.asarray(link, dtype=np.uint8))
+        ids = None if id_of_col is None else np.ascontiguousarray(id_of_col, dtype=np.int64)
+        pr = SiteProblem()
+        pr.D, pr.k, pr.n_cols = D, int(k), n_cols
+        pr.F = _p(F, _c_f32p)
+        pr.drift_of_col = _p(drift, _c_i32p)
+        pr.id_of_col = _p(ids, _c_i64p)
+        pr.n_drift = link.shape[0]
+        lk = np.ascontiguousarray(link.reshape(-1, order="F"))
+        pr.link = _p(lk, _c_u8p)
+        pr.in_begin, pr.n_in = int(in_begin), int(n_in)
+        return pr, (F, drift, ids, lk)
+
+    def knn_drift(self, F, drift_of_col, link, in_begin, n_in, k, id_of_col=None, opts=None):
+        """F: (n_cols, D) float32 rows = MATLAB D x n_cols columns. Returns (dk[n_in], knn[n_in, k])."""
+        pr, keep = self._site_problem(F, drift_of_col, link, in_begin, n_in, k, id_of_col)
+        dk = np.zeros(int(n_in), np.float32)
+        knn = np.zeros((int(n_in), int(k)), np.int32)
+        o = opts or make_options()
+        self._check(self.lib.irc_knn_drift(self.h, ctypes.byref(pr), ctypes.byref(o), _p(dk, _c_f32p),
+                                           _p(knn, _c_i32p)))
+        return dk, knn
+
+    def delta_drift(self, F, drift_of_col, link, in_begin, n_in, rho_cols, id_of_col=None, opts=None):
+        pr, keep = self._site_problem(F, drift_of_col, link, in_begin, n_in, 1, id_of_col)
+        rho = np.ascontiguousarray(rho_cols, dtype=np.float32)
+        dmin = np.zeros(int(n_in), np.float32)
+        nn = np.zeros(int(n_in), np.int32)
+        o = opts or make_options()
+        self._check(self.lib.irc_delta_drift(self.h, ctypes.byref(pr), ctypes.byref(o), _p(rho, _c_f32p),
+                                             _p(dmin, _c_f32p), _p(nn, _c_i32p)))
+        return dmin, nn
+
+    # ---- batch form (sort_page_) ------------------------------------------------
+    def rho_delta_batch(self, layout, k, opts=None, want_knn=True, want_dk=False):
+        """layout: dict from ironclust_amd.irc2.site_layout (host numpy arrays)."""
+        F = np.ascontiguousarray(layout["F"], dtype=np.float32)
+        col_spike = np.ascontiguousarray(layout["col_spike"], dtype=np.int32)
+        site_ptr = np.ascontiguousarray(layout["site_ptr"], dtype=np.int64)
+        site_np = np.ascontiguousarray(layout["site_n_primary"], dtype=np.int64)
+        lim = np.ascontiguousarray(layout["lim_drift"], dtype=np.int64)
+        link = np.asarray(layout["link"], dtype=np.uint8)
+        lk = np.ascontiguousarray(link.reshape(-1, order="F"))
+        N = int(layout["n_spikes"])
+        b = Batch()
+        b.n_spikes, b.D, b.k = N, F.shape[1], int(k)
+        b.n_sites, b.n_drift, b.n_cols = len(site_np), len(lim) - 1, F.shape[0]
+        b.F, b.col_spike = _p(F, _c_f32p), _p(col_spike, _c_i32p)
+        b.site_ptr, b.site_n_primary = _p(site_ptr, _c_i64p), _p(site_np, _c_i64p)
+        b.lim_drift, b.link = _p(lim, _c_i64p), _p(lk, _c_u8p)
+        rho = np.zeros(N, np.float32)
+        delta = np.zeros(N, np.float32)
+        nn = np.zeros(N, np.int64)
+        knn = np.zeros((N, int(k)), np.int64) if want_knn else None
+        dk = np.zeros(N, np.float32) if want_dk else None
+        out = Outputs(_p(rho, _c_f32p), _p(delta, _c_f32p), _p(nn, _c_i64p), _p(knn, _c_i64p), _p(dk, _c_f32p))
+        o = opts or make_options()
+        o.mem = MEM_HOST
+        self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(out)))
+        res = {"rho": rho, "delta": delta, "nneigh": nn}
+        if want_knn:
+            res["knn"] = knn
+        if want_dk:
+            res["dk"] = dk
+        return res
+
+    def rho_delta_batch_device(self, dev, out, opts=None):
+        """Device-resident form.  dev: dict of torch tensors on the context's device
+        (F [n_cols, D] float32, col_spike int32) plus host arrays site_ptr, site_n_primary,
+        lim_drift, link, n_spikes, k.  out: dict of torch device tensors rho, delta, nneigh
+        (int64), optional knn (int64 [N, k]) and dk."""
+        site_ptr = np.ascontiguousarray(dev["site_ptr"], dtype=np.int64)
+        site_np = np.ascontiguousarray(dev["site_n_primary"], dtype=np.int64)
+        lim = np.ascontiguousarray(dev["lim_drift"], dtype=np.int64)
+        lk = np.ascontiguousarray(np.asarray(dev["link"], dtype=np.uint8).reshape(-1, order="F"))
+        F = dev["F"]
+        b = Batch()
+        b.n_spikes, b.D, b.k = int(dev["n_spikes"]), int(F.shape[1]), int(dev["k"])
+        b.n_sites, b.n_drift, b.n_cols = len(site_np), len(lim) - 1, int(F.shape[0])
+        b.F, b.col_spike = _dptr(F, _c_f32p), _dptr(dev["col_spike"], _c_i32p)
+        b.site_ptr, b.site_n_primary = _p(site_ptr, _c_i64p), _p(site_np, _c_i64p)
+        b.lim_drift, b.link = _p(lim, _c_i64p), _p(lk, _c_u8p)
+        o = opts or make_options()
+        o.mem = MEM_DEVICE
+        outs = Outputs(_dptr(out["rho"], _c_f32p), _dptr(out["delta"], _c_f32p), _dptr(out["nneigh"], _c_i64p),
+                       _dptr(out.get("knn"), _c_i64p), _dptr(out.get("dk"), _c_f32p))
+        self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(outs)))
+
+    # ---- legacy A/B and dc forms ------------------------------------------------------
+    def knn_ab(self, B, A, k, want_index=True):
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        K = np.zeros(A.shape[0], np.float32)
+        I = np.zeros((A.shape[0], int(k)), np.int32) if want_index else None
+        self._check(self.lib.irc_knn_ab(self.h, B.shape[1], B.shape[0], _p(B, _c_f32p), A.shape[0],
+                                        _p(A, _c_f32p), int(k), _p(K, _c_f32p), _p(I, _c_i32p)))
+        return K, I
+
+    def delta_knn_ab(self, B, RB, A, RA):
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        RB = np.ascontiguousarray(RB, dtype=np.float32)
+        RA = np.ascontiguousarray(RA, dtype=np.float32)
+        Dd = np.zeros(A.shape[0], np.float32)
+        Nn = np.zeros(A.shape[0], np.int32)
+        self._check(self.lib.irc_delta_knn_ab(self.h, B.shape[1], B.shape[0], _p(B, _c_f32p), _p(RB, _c_f32p),
+                                              A.shape[0], _p(A, _c_f32p), _p(RA, _c_f32p), _p(Dd, _c_f32p),
+                                              _p(Nn, _c_i32p)))
+        return Dd, Nn
+
+    def rho_drift_dc(self, n1, miSite12, fet1, fet2, dc2):
+        m = np.asfortranarray(np.asarray(miSite12, dtype=np.int32))
+        ms = np.ascontiguousarray(m.reshape(-1, order="F"))
+        f1 = np.ascontiguousarray(fet1, dtype=np.float32)
+        f2 = np.ascontiguousarray(fet2, dtype=np.float32)
+        rho = np.zeros(int(n1), np.float32)
+        self._check(self.lib.irc_rho_drift_dc(self.h, f1.shape[1], int(n1), f1.shape[0], _p(ms, _c_i32p),
+                                              _p(f1, _c_f32p), _p(f2, _c_f32p), float(dc2), _p(rho, _c_f32p)))
+        return rho
+
+    def delta_drift_dc(self, n1, miSite12, fet1, fet2, rho12, dc2):
+        m = np.asfortranarray(np.asarray(miSite12, dtype=np.int32))
+        ms = np.ascontiguousarray(m.reshape(-1, order="F"))
+        f1 = np.ascontiguousarray(fet1, dtype=np.float32)
+        f2 = np.ascontiguousarray(fet2, dtype=np.float32)
+        r = np.ascontiguousarray(rho12, dtype=np.float32)
+        dl = np.zeros(int(n1), np.float32)
+        nn = np.zeros(int(n1), np.uint32)
+        self._check(self.lib.irc_delta_drift_dc(self.h, f1.shape[1], int(n1), f1.shape[0], _p(ms, _c_i32p),
+                                                _p(f1, _c_f32p), _p(f2, _c_f32p), _p(r, _c_f32p), float(dc2),
+                                                _p(dl, _c_f32p), _p(nn, _c_u32p)))
+        return dl, nn

@@ -1,0 +1,73 @@
+"""Small independent restatements used to pin the oracle (numpy brute force on inputs whose
+squared distances are exact in any summation order, i.e. small integer features)."""
+import numpy as np
+
+
+def brute_batch(ds):
+    """Brute-force canonical outputs for integer-valued features (exact arithmetic)."""
+    N, k = ds["n_spikes"], ds["k"]
+    F1 = np.asarray(ds["F1"], np.float64)
+    F2 = None if ds.get("F2") is None else np.asarray(ds["F2"], np.float64)
+    s1 = np.asarray(ds["site1"])
+    s2 = None if ds.get("site2") is None else np.asarray(ds["site2"])
+    lim = np.asarray(ds["lim"])
+    link = np.asarray(ds["link"]).astype(bool)
+    chunk = np.searchsorted(lim, np.arange(1, N + 1), side="right") - 1
+    ids = np.arange(N)
+    cand = []
+    for a in range(N):
+        rows = link[:, chunk[a]]
+        m1 = (s1 == s1[a]) & rows[chunk]
+        b1 = ids[m1]
+        d1 = ((F1[b1] - F1[a]) ** 2).sum(1)
+        if s2 is not None:
+            m2 = (s2 == s1[a]) & rows[chunk]
+            b2 = ids[m2]
+            d2 = ((F2[b2] - F1[a]) ** 2).sum(1)
+            b1 = np.concatenate([b1, b2])
+            d1 = np.concatenate([d1, d2])
+        cand.append((b1, d1))
+    knn = np.zeros((N, k), np.int64)
+    rho = np.zeros(N, np.float32)
+    for a, (b, d) in enumerate(cand):
+        o = np.lexsort((b, d))
+        kk = min(k, len(o))
+        sel = list(o[:kk]) + [o[kk - 1]] * (k - kk)
+        knn[a] = b[sel] + 1
+        rho[a] = np.float32(1.0 / np.sqrt(d[o[kk - 1]]))
+    delta = np.zeros(N, np.float32)
+    nn = np.zeros(N, np.int64)
+    sent = np.float32(np.sqrt(3.402e38))
+    for a, (b, d) in enumerate(cand):
+        m = rho[b] > rho[a]
+        if not m.any():
+            delta[a], nn[a] = sent, a + 1
+            continue
+        bb, dd = b[m], d[m]
+        o = np.lexsort((bb, dd))[0]
+        v = np.float32(np.sqrt(dd[o]) * np.float64(rho[a]))
+        if not np.isfinite(v):
+            delta[a], nn[a] = sent, a + 1
+        else:
+            delta[a], nn[a] = v, bb[o] + 1
+    return dict(rho=rho, delta=delta, nneigh=nn, knn=knn)
+
+
+def int_dataset(N, S, C, D, k, seed, lo=-3, hi=4, f2=False, link=None, lim=None):
+    """Integer-valued features (many exact ties) with random sites and chunk links."""
+    rng = np.random.default_rng(seed)
+    F1 = rng.integers(lo, hi, size=(N, D)).astype(np.float32)
+    s1 = rng.integers(1, S + 1, size=N).astype(np.int32)
+    if lim is None:
+        c = np.arange(1, C + 1)
+        lim = np.concatenate([[0], np.ceil(c / C * N).astype(np.int64)]) + 1
+    if link is None:
+        link = (rng.random((C, C)) < 0.5).astype(np.uint8)
+        np.fill_diagonal(link, 1)
+    ds = dict(n_spikes=N, n_sites=S, D=D, n_drift=C, k=k, F1=F1, F2=None, site1=s1, site2=None,
+              lim=np.asarray(lim, np.int64), link=np.asarray(link, np.uint8))
+    if f2:
+        s2 = np.where(rng.random(N) < 0.5, (s1 + rng.integers(1, S, size=N) - 1) % S + 1, 0).astype(np.int32)
+        ds["site2"] = s2
+        ds["F2"] = rng.integers(lo, hi, size=(N, D)).astype(np.float32)
+    return ds

@@ -1,0 +1,196 @@
+"""GPU parity: the gfx950 library against the oracle through the C ABI, bit-exact for
+indices, rho, delta and nneigh (the canonical values are defined in FP64 and the library
+reports FP64-derived values whatever its screen precision)."""
+import numpy as np
+import pytest
+
+from ironclust_amd import irc2, libirc, synth
+from oracle import oracle as O
+from tests.helpers import int_dataset
+
+pytestmark = pytest.mark.gpu
+
+PRECS = ["exact", "fp32", "bf16"]
+
+
+def _check_batch(ds, prec, ctxname=""):
+    lay = irc2.site_layout({kk: (__import__("torch").as_tensor(v) if kk in ("F1", "F2", "site1", "site2")
+                                 and v is not None else v) for kk, v in ds.items()})
+    ref = O.sort_batch(ds)
+    rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": ds["k"], "precision": prec})
+    np.testing.assert_array_equal(knn.T, ref["knn"], err_msg=f"knn {ctxname} {prec}")
+    np.testing.assert_array_equal(rho, ref["rho"], err_msg=f"rho {ctxname} {prec}")
+    np.testing.assert_array_equal(nn, ref["nneigh"], err_msg=f"nneigh {ctxname} {prec}")
+    np.testing.assert_array_equal(delta, ref["delta"], err_msg=f"delta {ctxname} {prec}")
+    return ref
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_drift_with_secondary_sites(prec):
+    ds = synth.make_dataset(12000, 5, 14, 10, 4, k=16, drift_step=0.3, n_links=4, band=3, seed=31, f2=True)
+    _check_batch(ds, prec, "drift-f2")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_plain_single_chunk(prec):
+    ds = synth.make_dataset(8000, 3, 24, 1, 8, k=30, seed=32)
+    _check_batch(ds, prec, "plain")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_integer_ties(prec):
+    # integer features: many exactly tied distances and duplicate points (d2 = 0, rho = inf)
+    ds = int_dataset(N=1500, S=3, C=6, D=3, k=9, seed=33, f2=True, lo=-2, hi=3)
+    _check_batch(ds, prec, "ties")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_d42_k30(prec):
+    ds = synth.make_dataset(30000, 4, 42, 20, 10, k=30, drift_step=0.3, n_links=15, band=8, seed=34)
+    _check_batch(ds, prec, "d42")
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_batch_fewer_candidates_than_k(prec):
+    # sparse sites: many (site, chunk) blocks hold fewer than k candidates, some none
+    ds = synth.make_dataset(600, 40, 6, 12, 2, k=20, drift_step=0.2, n_links=2, band=1, seed=35)
+    _check_batch(ds, prec, "sparse")
+
+
+def test_batch_odd_dims_and_k():
+    for D, k in ((1, 1), (5, 3), (45, 33), (62, 64)):
+        ds = synth.make_dataset(3000, 2, D, 3, 3, k=k, drift_step=0.1, n_links=2, band=1, seed=36 + D)
+        for prec in PRECS:
+            _check_batch(ds, prec, f"D{D}k{k}")
+
+
+def test_site_entry_points(ctx):
+    ds = int_dataset(N=900, S=1, C=5, D=4, k=8, seed=41, f2=False, lo=-3, hi=4)
+    F = np.asarray(ds["F1"])
+    drift = np.searchsorted(ds["lim"], np.arange(1, 901), side="right").astype(np.int32)
+    for (a, n) in ((0, 900), (100, 300), (899, 1)):
+        dk_r, I_r = O.knn_site(F, drift, ds["link"], a, n, 8)
+        dk_g, I_g = ctx.knn_drift(F, drift, ds["link"], a, n, 8, opts=libirc.make_options(libirc.PREC_EXACT))
+        np.testing.assert_array_equal(I_g, I_r)
+        np.testing.assert_array_equal(dk_g, dk_r)
+    rho = np.random.default_rng(0).random(900).astype(np.float32)
+    dm_r, nn_r = O.delta_site(F, drift, ds["link"], 50, 700, rho)
+    dm_g, nn_g = ctx.delta_drift(F, drift, ds["link"], 50, 700, rho)
+    np.testing.assert_array_equal(nn_g, nn_r)
+    np.testing.assert_array_equal(dm_g, dm_r)
+
+
+def test_site_entry_unsorted_drift_and_ids(ctx):
+    # columns in arbitrary drift order with explicit spike ids for tie-breaking
+    rng = np.random.default_rng(5)
+    n = 700
+    F = rng.integers(-2, 3, size=(n, 3)).astype(np.float32)
+    drift = rng.integers(1, 5, size=n).astype(np.int32)
+    ids = rng.permutation(n).astype(np.int64) + 1000
+    link = (rng.random((4, 4)) < 0.6).astype(np.uint8)
+    np.fill_diagonal(link, 1)
+    dk_r, I_r = O.knn_site(F, drift, link, 0, n, 6, id_of_col=ids)
+    dk_g, I_g = ctx.knn_drift(F, drift, link, 0, n, 6, id_of_col=ids)
+    np.testing.assert_array_equal(I_g, I_r)
+    np.testing.assert_array_equal(dk_g, dk_r)
+
+
+def test_mirror_functions_match_reference_orientation():
+    ds = synth.make_dataset(2000, 1, 9, 4, 3, k=10, drift_step=0.3, n_links=2, band=1, seed=43)
+    F = ds["F1"].numpy()
+    drift = np.searchsorted(ds["lim"], np.arange(1, 2001), side="right").astype(np.int32)
+    vl = np.zeros(2000, bool)
+    vl[500:1500] = True
+    vrKnn, miKnn = irc2.search_knn_drift_(vl, F.T, drift, ds["link"].astype(bool), {"knn": 10})
+    dk_r, I_r = O.knn_site(F, drift, ds["link"], 500, 1000, 10)
+    assert miKnn.shape == (10, 1000)
+    np.testing.assert_array_equal(miKnn.T, I_r)
+    np.testing.assert_array_equal(vrKnn, dk_r)
+    rho = (1.0 / np.maximum(vrKnn, 1e-30)).astype(np.float32)
+    rho_out = np.random.default_rng(1).random(2000).astype(np.float32)
+    rho_out[500:1500] = rho
+    d_g, n_g = irc2.search_delta_drift_(vl, F.T, rho_out, drift, ds["link"].astype(bool), {"knn": 10})
+    d_r, n_r = O.delta_site(F, drift, ds["link"], 500, 1000, rho_out)
+    np.testing.assert_array_equal(n_g, n_r)
+    np.testing.assert_array_equal(d_g, d_r)
+    K, fGpu, I = irc2.cuda_knn_(F[:300].T, F[300:400].T, {"knn": 7})
+    K_r, I_r = O.knn_ab(F[:300], F[300:400], 7)
+    np.testing.assert_array_equal(I.T, I_r)
+    np.testing.assert_array_equal(K, K_r)
+
+
+def test_ab_and_dc_entry_points(ctx):
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((500, 11)).astype(np.float32)
+    A = rng.standard_normal((200, 11)).astype(np.float32)
+    for k in (1, 5, 30):
+        K_g, I_g = ctx.knn_ab(B, A, k)
+        K_r, I_r = O.knn_ab(B, A, k)
+        np.testing.assert_array_equal(I_g, I_r)
+        np.testing.assert_array_equal(K_g, K_r)
+    K_g, I_n = ctx.knn_ab(B[:3], A, 5, want_index=False)
+    np.testing.assert_array_equal(K_g, O.knn_ab(B[:3], A, 5)[0])
+    RB = rng.random(500).astype(np.float32)
+    RA = rng.random(200).astype(np.float32)
+    np.testing.assert_array_equal(np.stack(ctx.delta_knn_ab(B, RB, A, RA)), np.stack(O.delta_knn_ab(B, RB, A, RA)))
+    n12, n1 = 3000, 1200
+    site = np.stack([rng.integers(1, 6, n12), rng.integers(1, 6, n12)], 1).astype(np.int32)
+    f1 = rng.standard_normal((n12, 9)).astype(np.float32)
+    f2 = rng.standard_normal((n12, 9)).astype(np.float32)
+    rho_g = ctx.rho_drift_dc(n1, site, f1, f2, 6.0)
+    rho_r = O.rho_drift_dc(n1, site, f1, f2, 6.0)
+    np.testing.assert_array_equal(rho_g, rho_r)
+    rho12 = np.concatenate([rho_r, rng.random(n12 - n1).astype(np.float32)])
+    dl_g, nn_g = ctx.delta_drift_dc(n1, site, f1, f2, rho12, 6.0)
+    dl_r, nn_r = O.delta_drift_dc(n1, site, f1, f2, rho12, 6.0)
+    np.testing.assert_array_equal(nn_g, nn_r)
+    np.testing.assert_array_equal(dl_g, dl_r)
+
+
+def test_golden_fixtures():
+    import glob
+    import os
+    files = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+    assert files
+    for f in files:
+        z = np.load(f)
+        if "site_ptr" not in z:
+            continue
+        lay = dict(F=z["F"], col_spike=z["col_spike"], site_ptr=z["site_ptr"], site_n_primary=z["site_n_primary"],
+                   lim_drift=z["lim"], link=z["link"], n_spikes=int(z["n_spikes"]), k=int(z["k"]))
+        for prec in PRECS:
+            rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": int(z["k"]), "precision": prec})
+            np.testing.assert_array_equal(knn.T, z["knn"], err_msg=f)
+            np.testing.assert_array_equal(rho, z["rho"], err_msg=f)
+            np.testing.assert_array_equal(delta, z["delta"], err_msg=f)
+            np.testing.assert_array_equal(nn, z["nneigh"], err_msg=f)
+
+
+def test_force_fail_and_errors(ctx):
+    ds = synth.make_dataset(500, 2, 4, 2, 2, k=5, seed=50)
+    lay = irc2.site_layout(ds)
+    host = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in lay.items()}
+    with pytest.raises(libirc.IrcError) as e:
+        ctx.rho_delta_batch(host, 5, opts=libirc.make_options(force_fail=True))
+    assert e.value.status == libirc.IRC_E_FORCED
+    # a bad descriptor is rejected cleanly and the context stays usable
+    bad = dict(host, site_n_primary=host["site_n_primary"] + 1)
+    with pytest.raises(libirc.IrcError) as e:
+        ctx.rho_delta_batch(bad, 5)
+    assert e.value.status == libirc.IRC_E_ARG
+    with pytest.raises(libirc.IrcError):
+        ctx.rho_delta_batch(host, 65)          # k above the supported maximum
+    r = ctx.rho_delta_batch(host, 5)
+    ref = O.sort_batch(ds)
+    np.testing.assert_array_equal(r["rho"], ref["rho"])
+
+
+def test_fp64_sqrt_div_rounding_matches_host(ctx):
+    # rho = float(1/sqrt(d2)) relies on correctly rounded FP64 sqrt and division on the GPU;
+    # the lattice-free A/B form with one candidate reports float(sqrt(d2)) for random d2.
+    rng = np.random.default_rng(9)
+    A = (rng.standard_normal((4000, 3)) * rng.choice([1e-3, 1.0, 1e3], size=(4000, 1))).astype(np.float32)
+    B = np.zeros((1, 3), np.float32)
+    K_g, _ = ctx.knn_ab(B, A, 1)
+    K_r, _ = O.knn_ab(B, A, 1)
+    np.testing.assert_array_equal(K_g, K_r)
