@@ -175,10 +175,15 @@ typedef struct {
     double ms_total;          /* device time of the whole call, HIP events */
     double ms_knn;            /* kNN + rho kernels */
     double ms_delta;          /* delta kernels */
+    double ms_screen;         /* the screen kernel alone (0 on the exact path) */
     int32_t precision_used;
 } irc_stats;
 
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);
+
+/* Self-test of the MFMA lane layouts and the cross-half lane swap the screen kernel relies
+ * on (no reference counterpart: a diagnostic).  out: 4096 floats, host memory. */
+irc_status irc_selftest_mfma(irc_ctx* ctx, float* out);
 
 /* ---------------------------------------------------------------------------
  * Legacy A/B forms (irc v4 path).

@@ -280,32 +280,125 @@ int resolve_precision(const irc_options* opt) {
     return prec;
 }
 
-// kNN pass over a device plan: exact path or screen + certification + exact fallback.
-irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int* knn_col, double* knn_t,
-                   double* dk2) {
+// Query range of one (site, chunk) block: queries q0 .. q0+nq-1 share the candidate
+// stream of the block's linked chunks (ncand columns in nruns runs).
+struct QRange {
+    int blk, q0, nq, ncand, nruns;
+};
+
+void block_ranges_cands(int C, const std::vector<int>& runl, const std::vector<int>& lptr,
+                        const std::vector<int>& lidx, std::vector<QRange>& rg) {
+    for (auto& r : rg) {
+        const int s = r.blk / C, c = r.blk % C;
+        int n = 0, nr = 0;
+        for (int li = lptr[c]; li < lptr[c + 1]; ++li) {
+            const int b2 = s * C + lidx[li];
+            for (int part = 0; part < 2; ++part) {
+                const int len = runl[2 * b2 + part];
+                if (len > 0) { n += len; ++nr; }
+            }
+        }
+        r.ncand = n;
+        r.nruns = nr;
+    }
+}
+
+// kNN pass: MFMA screen + certification + FP64 re-rank for every block that fits the
+// screen, the exact FP64 kernel for the rest and for uncertified queries.
+irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, const float* F, int FS,
+                   int64_t n_cols, const std::vector<QRange>& ranges, int* knn_col, double* knn_t, double* dk2) {
     hipStream_t st = ctx->stream;
+    int64_t pairs = 0;
+    for (const auto& r : ranges) pairs += (int64_t)r.nq * r.ncand;
+    ctx->stats.n_pairs = pairs;
     ctx->stats.n_screen_fallback = 0;
-    if (prec == IRC_PREC_EXACT || !irc::screen_supported(p)) {
+    const bool screen = prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN;
+    if (!screen) {
         ctx->stats.precision_used = IRC_PREC_EXACT;
+        HIPCHK(hipEventRecord(ctx->ev[4], st));
         HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2, nullptr));
+        HIPCHK(hipEventRecord(ctx->ev[5], st));
+        ctx->stats.n_screen_fallback = nq;
         return IRC_OK;
     }
     ctx->stats.precision_used = prec;
-    irc::ScreenBufs sb;
-    CHK(ensure(ctx, ctx->b_scr[0], irc::screen_pack_bytes(p, ctx->stats.n_queries, prec)));
-    CHK(ensure(ctx, ctx->b_scr[1], sizeof(int) * (size_t)nq + 64));
-    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)p.C * 0 + irc::screen_aux_bytes(p)));
-    sb.pack = ctx->b_scr[0].p;
-    sb.fallback = ptr<int>(ctx->b_scr[1]);
-    sb.n_fallback = ptr<int>(ctx->b_scr[1]) + nq;  // counter after the list
-    sb.aux = ctx->b_scr[2].p;
-    HIPCHK(hipMemsetAsync(sb.n_fallback, 0, sizeof(int), st));
-    HIPCHK(irc::launch_screen_knn(st, p, nq, prec, k_over, sb, ctx->n_cu, knn_col, knn_t, dk2));
-    // exact kNN for queries whose screened candidate set could not be certified
-    HIPCHK(irc::launch_knn_exact_dev(st, p, sb.fallback, sb.n_fallback, nq, persistent_grid(ctx, 8), knn_col,
-                                     knn_t, dk2));
+    const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : irc::SCREEN_BF16;
+    int kprime = k_over <= 0 ? 32 : k_over;
+    if (kprime < p.k) kprime = p.k;
+    kprime = kprime <= 32 ? 32 : (kprime <= 48 ? 48 : 64);
+    // items (<= 64 queries each) in site-major order; queries outside the screen's limits go exact
+    std::vector<irc::ScreenItem> items;
+    std::vector<int> exact_q;
+    std::vector<double> cost;
+    for (const auto& r : ranges) {
+        if (r.nq <= 0) continue;
+        if (r.ncand > IRC_SCREEN_MAXCAND || r.nruns > IRC_SCREEN_MAXRUNS) {
+            for (int i = 0; i < r.nq; ++i) exact_q.push_back(r.q0 + i);
+            continue;
+        }
+        for (int i = 0; i < r.nq; i += 64) {
+            const int n = r.nq - i < 64 ? r.nq - i : 64;
+            items.push_back({r.blk, r.q0 + i, n, r.ncand});
+            cost.push_back((double)r.ncand + 64.0);
+        }
+    }
+    // eight contiguous segments of about equal cost, one per XCD
+    int seg_beg[9];
+    {
+        double tot = 0.0;
+        for (double c : cost) tot += c;
+        double acc = 0.0;
+        int sgi = 1;
+        seg_beg[0] = 0;
+        for (size_t i = 0; i < items.size() && sgi < 8; ++i) {
+            acc += cost[i];
+            while (sgi < 8 && acc >= tot * sgi / 8.0) seg_beg[sgi++] = (int)i + 1;
+        }
+        while (sgi < 8) seg_beg[sgi++] = (int)items.size();
+        seg_beg[8] = (int)items.size();
+    }
+    const int KP = irc::screen_kp(p.D);
+    const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
+    CHK(ensure(ctx, ctx->b_scr[0], eb * (size_t)n_cols * KP));
+    CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
+    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols));
+    CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
+    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
+    CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
+    int* fb = ptr<int>(ctx->b_scr[4]);
+    int* nfb = fb + nq;
+    int* seg_ctr = ptr<int>(ctx->b_scr[5]);
+    int* d_seg_beg = seg_ctr + 8;
+    const int n_exact = (int)exact_q.size();
+    if (!items.empty())
+        HIPCHK(hipMemcpyAsync(ctx->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
+                              hipMemcpyHostToDevice, st));
+    if (n_exact) HIPCHK(hipMemcpyAsync(fb, exact_q.data(), sizeof(int) * n_exact, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(nfb, &n_exact, sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_seg_beg, seg_beg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
+                                   ptr<float>(ctx->b_scr[2])));
+    irc::ScreenArgs a;
+    a.p = p;
+    a.hi = ctx->b_scr[0].p;
+    a.lo = ctx->b_scr[1].p;
+    a.norm = ptr<float>(ctx->b_scr[2]);
+    a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
+    a.n_items = (int)items.size();
+    a.kprime = kprime;
+    a.eta = irc::screen_eta(sp, p.D, KP);
+    a.knn_col = knn_col;
+    a.knn_t = knn_t;
+    a.dk2 = dk2;
+    a.fallback = fb;
+    a.n_fallback = nfb;
+    HIPCHK(hipEventRecord(ctx->ev[4], st));
+    HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, 8), seg_ctr, d_seg_beg));
+    HIPCHK(hipEventRecord(ctx->ev[5], st));
+    // exact kNN for queries outside the screen's limits and for uncertified queries
+    HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2));
     int nf = 0;
-    HIPCHK(hipMemcpyAsync(&nf, sb.n_fallback, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&nf, nfb, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.n_screen_fallback = nf;
     return IRC_OK;
@@ -376,6 +469,17 @@ void irc_ctx_destroy(irc_ctx* ctx) {
 
 const char* irc_last_error(const irc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+irc_status irc_selftest_mfma(irc_ctx* ctx, float* out) {
+    if (!ctx || !out) return IRC_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ensure(ctx, ctx->b_scr[7], sizeof(float) * 4096));
+    hipStream_t st = ctx->stream;
+    HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->b_scr[7])));
+    HIPCHK(hipMemcpyAsync(out, ctx->b_scr[7].p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st) {
     if (!ctx || !st) return IRC_E_ARG;
     *st = ctx->stats;
@@ -404,9 +508,18 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
+    // query ranges: maximal runs of queries in the same drift chunk
+    std::vector<QRange> ranges;
+    for (int i = 0; i < nq;) {
+        int j = i;
+        while (j < nq && h.qblk[j] == h.qblk[i]) ++j;
+        ranges.push_back({h.qblk[i], i, j - i, 0, 0});
+        i = j;
+    }
+    block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, ptr<int>(ctx->b_knncol),
-                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, p.feat, p.DP, h.n_cols, ranges,
+                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
@@ -414,6 +527,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     HIPCHK(hipMemcpyAsync(knn_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.ms_knn = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
     return IRC_OK;
 }
 
@@ -588,9 +702,23 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
     if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
     if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
+    // per-block query ranges and candidate counts (the block runs computed on the device)
+    std::vector<int> runb((size_t)S * C * 2), runl((size_t)S * C * 2);
+    HIPCHK(hipMemcpyAsync(runb.data(), ctx->b_runb.p, sizeof(int) * runb.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(runl.data(), ctx->b_runl.p, sizeof(int) * runl.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<QRange> ranges;
+    ranges.reserve((size_t)S * C);
+    for (int s = 0; s < S; ++s)
+        for (int c = 0; c < C; ++c) {
+            const int bb = s * C + c;
+            if (runl[2 * bb] == 0) continue;
+            ranges.push_back({bb, (int)(prim[s] + (runb[2 * bb] - b->site_ptr[s])), runl[2 * bb], 0, 0});
+        }
+    block_ranges_cands(C, runl, lptr, lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, ptr<int>(ctx->b_knncol),
-                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, dF, D, n_cols, ranges,
+                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk, o_knn,
                                   nullptr));
     HIPCHK(hipEventRecord(ctx->ev[2], st));
@@ -611,6 +739,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     }
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.n_delta_full = nunres;
+    ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
     ctx->stats.ms_knn = event_ms(ctx->ev[1], ctx->ev[2]);
     ctx->stats.ms_delta = event_ms(ctx->ev[2], ctx->ev[3]);
     ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[3]);
@@ -640,8 +769,8 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
-    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
-                ptr<double>(ctx->b_dk2)));
+    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, p.feat, p.DP, h.n_cols, {}, ptr<int>(ctx->b_knncol),
+                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));

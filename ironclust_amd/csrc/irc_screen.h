@@ -4,17 +4,44 @@
 
 namespace irc {
 
-struct ScreenBufs {
-    void* pack = nullptr;     // packed candidate operands (fp32 or bf16 hi/lo rows)
-    void* aux = nullptr;      // per-block norms / bounds
-    int* fallback = nullptr;  // queries whose candidate set was not certified
-    int* n_fallback = nullptr;
+// One screen work item: up to 64 queries of one (site, chunk) block against the
+// block's whole candidate stream (the runs of the linked chunks).
+struct ScreenItem {
+    int blk;     // site * C + chunk
+    int q0;      // first query index (queries of a block are contiguous)
+    int nq;      // queries in this item (1..64)
+    int ncand;   // candidates of the block (<= IRC_SCREEN_MAXCAND)
 };
 
-bool screen_supported(const DevPlan& p);
-size_t screen_pack_bytes(const DevPlan& p, int64_t n_cols, int prec);
-size_t screen_aux_bytes(const DevPlan& p);
-hipError_t launch_screen_knn(hipStream_t st, const DevPlan& p, int nq, int prec, int k_over, const ScreenBufs& sb,
-                             int n_cu, int* knn_col, double* knn_t, double* dk2);
+#define IRC_SCREEN_MAXCAND 8192   // candidate positions fit 13 bits of the packed key
+#define IRC_SCREEN_MAXRUNS 128    // runs per block: 2 per linked chunk
+
+struct ScreenArgs {
+    DevPlan p;
+    const void* hi;          // [n_cols][KP] bf16 bits (bf16 form) or fp32 rows (fp32 form)
+    const void* lo;          // [n_cols][KP] bf16 low parts (bf16 form)
+    const float* norm;       // [n_cols] fp32 |x|^2
+    const ScreenItem* items;
+    int n_items;
+    int kprime;              // over-fetched candidates per query (32, 48 or 64)
+    double eta;              // |screened - exact| <= eta * (|q|^2 + |c|^2)
+    int* knn_col;
+    double* knn_t;
+    double* dk2;
+    int* fallback;           // queries to redo with the exact kernel
+    int* n_fallback;
+};
+
+enum { SCREEN_FP32 = 0, SCREEN_BF16 = 1 };
+
+int screen_kp(int D);                                  // padded operand width (multiple of 16)
+double screen_eta(int prec, int D, int KP);            // rigorous error factor
+hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
+                              void* hi, void* lo, float* norm);
+// seg_ctr: 8 device counters (zeroed by the launch); seg_beg: 9 device offsets of the
+// per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
+hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,
+                               const int* seg_beg);
+hipError_t launch_mfma_layout_probe(hipStream_t st, float* out);
 
 }  // namespace irc

@@ -1,15 +1,516 @@
-// irc_screen.hip -- MFMA distance screen for the kNN pass (gfx950).
-#include "irc_screen.h"
+// irc_screen.hip -- MFMA distance screen of the kNN pass for gfx950.
+//
+// The screen replaces the per-thread binned-minimum search of search_min_drift.cu:24-94
+// (and the host sort of its nThreads x nA output, matlab/irc2.m:3784-3787) by:
+//   1. distance tiles as a dense contraction on MFMA: with candidate rows
+//      c~ = [c, 1, |c|^2] and query rows q~ = [-2q, |q|^2, 1], one 32x32 product gives
+//      32 candidates x 32 queries of |q|^2 + |c|^2 - 2 q.c directly;
+//        FP32 form: v_mfma_f32_32x32x2_f32
+//        BF16 form: v_mfma_f32_32x32x16_bf16 on a hi/lo split of every operand
+//                   (x = hi + lo + r, |r| <= 2^-16 |x|), products hi*hi + hi*lo + lo*hi,
+//                   accumulated in FP32;
+//   2. a per-query list of the K' smallest screened keys (32-bit keys: truncated
+//      distance bits above the candidate's stream position), kept sorted in VGPRs and
+//      refreshed from an LDS append buffer with bitonic networks;
+//   3. a rigorous bound |screened - exact| <= eta * (|q|^2 + max|c|^2) certifies that
+//      every candidate outside the list is farther than the k-th; the candidates that
+//      can still be in the top k are re-ranked with the canonical FP64 distance and
+//      ordered by (d2, spike id).  Uncertified queries go to the exact kernel.
+// A wavefront owns 64 queries of one (site, chunk) block: two 32-query column tiles
+// share each candidate tile; a lane permutation (v_permlane32_swap) gives every lane
+// all 32 distances of its own query.
+#include <hip/hip_runtime.h>
+
 #include "irc_kernels.h"
+#include "irc_screen.h"
+
+#pragma clang fp contract(off)
 
 namespace irc {
 
-bool screen_supported(const DevPlan&) { return false; }
-size_t screen_pack_bytes(const DevPlan&, int64_t, int) { return 0; }
-size_t screen_aux_bytes(const DevPlan&) { return 0; }
-hipError_t launch_screen_knn(hipStream_t, const DevPlan&, int, int, int, const ScreenBufs&, int, int*, double*,
-                             double*) {
-    return hipErrorNotSupported;
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef short short8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2bf_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// value of dimension d of the candidate-form row [x, 1, |x|^2, 0...]
+__device__ __forceinline__ float cand_form(const float* __restrict__ F, int D, int d, float n) {
+    return d < D ? F[d] : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
+}
+
+__global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
+                              void* __restrict__ hi, void* __restrict__ lo, float* __restrict__ norm) {
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n_cols) return;
+    const float* x = F + col * FS;
+    float n = 0.0f;
+    for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
+    norm[col] = n;
+    if (prec == SCREEN_FP32) {
+        float* r = reinterpret_cast<float*>(hi) + col * KP;
+        for (int d = 0; d < KP; ++d) r[d] = cand_form(x, D, d, n);
+    } else {
+        uint16_t* rh = reinterpret_cast<uint16_t*>(hi) + col * KP;
+        uint16_t* rl = reinterpret_cast<uint16_t*>(lo) + col * KP;
+        for (int d = 0; d < KP; ++d) {
+            const float v = cand_form(x, D, d, n);
+            const uint16_t h = f2bf_rne(v);
+            rh[d] = h;
+            rl[d] = f2bf_rne(v - bf2f(h));
+        }
+    }
+}
+
+// query-form value of dimension d from candidate-form row values: [-2x, |x|^2, 1, 0...]
+__device__ __forceinline__ int qsrc(int d, int D) { return d < D ? d : (d == D ? D + 1 : (d == D + 1 ? D : d)); }
+
+template <int N>
+__device__ __forceinline__ void cex(uint32_t& a, uint32_t& b, bool up) {
+    const uint32_t lo = a < b ? a : b;
+    const uint32_t hi = a < b ? b : a;
+    a = up ? lo : hi;
+    b = up ? hi : lo;
+}
+
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint32_t* v) {
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+                const int l = i ^ j;
+                if (l > i) cex<N>(v[i], v[l], (i & k) == 0);
+            }
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void bitonic_clean(uint32_t* v) {
+#pragma unroll
+    for (int j = N >> 1; j > 0; j >>= 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const int l = i ^ j;
+            if (l > i) cex<N>(v[i], v[l], true);
+        }
+    }
+}
+
+__device__ __forceinline__ int ceil_log2(int n) {
+    int b = 0;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+
+__device__ __forceinline__ double exact_d2(const float* __restrict__ q, const float* __restrict__ x, int DP) {
+    double acc = 0.0;
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    for (int d4 = 0; d4 < DP / 4; ++d4) {
+        const float4 a = q4[d4], b = x4[d4];
+        double t;
+        t = (double)a.x - (double)b.x; acc = fma(t, t, acc);
+        t = (double)a.y - (double)b.y; acc = fma(t, t, acc);
+        t = (double)a.z - (double)b.z; acc = fma(t, t, acc);
+        t = (double)a.w - (double)b.w; acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ float key_dist(uint32_t key, uint32_t hmask) { return __uint_as_float(key & hmask); }
+
+constexpr int WIN = 16;  // re-rank window: longest run of mutually ambiguous neighbours
+
+template <int PREC, int KB, int LS>
+__global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
+                                                   const int* __restrict__ seg_beg) {
+    constexpr int KP = 16 * KB;
+    __shared__ uint32_t sbuf[64 * 64];
+    __shared__ int s_rb[IRC_SCREEN_MAXRUNS + 1];
+    __shared__ int s_ro[IRC_SCREEN_MAXRUNS + 1];
+    __shared__ int s_nr, s_item;
+    const DevPlan& p = a.p;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5, m = lane & 31;
+    const int kprime = a.kprime;
+    // workgroups are dispatched round-robin over the 8 XCDs: blockIdx & 7 names the XCD
+    const int my_seg = blockIdx.x & 7;
+    for (;;) {
+        // ---- dequeue: own XCD's segment first (its blocks share candidate chunks in
+        // that XCD's L2), then the other segments; every wave ends when all are empty
+        __syncthreads();
+        if (lane == 0) {
+            int got = -1;
+            for (int t = 0; t < 8 && got < 0; ++t) {
+                const int sg = (my_seg + t) & 7;
+                if (seg_ctr[sg] >= seg_beg[sg + 1] - seg_beg[sg]) continue;
+                const int idx = atomicAdd(&seg_ctr[sg], 1);
+                if (seg_beg[sg] + idx < seg_beg[sg + 1]) got = seg_beg[sg] + idx;
+            }
+            s_item = got;
+        }
+        __syncthreads();
+        const int it = s_item;
+        if (it < 0) break;
+        const ScreenItem item = a.items[it];
+        const int s = item.blk / p.C, c = item.blk - s * p.C;
+        if (lane == 0) {
+            int nr = 0, off = 0;
+            for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
+                const int b2 = s * p.C + p.link_idx[li];
+                for (int part = 0; part < 2; ++part) {
+                    const int len = p.run_len[2 * b2 + part];
+                    if (len > 0) {
+                        s_rb[nr] = p.run_beg[2 * b2 + part];
+                        s_ro[nr] = off;
+                        off += len;
+                        ++nr;
+                    }
+                }
+            }
+            s_ro[nr] = off;
+            s_nr = nr;
+        }
+        __syncthreads();
+        const int nr = s_nr;
+        const int ncand = item.ncand;
+
+        // ---- query operands (B) for the two 32-query column tiles
+        bf16x8 bh[2][KB], bl[2][KB];
+        float4 bq[2][2 * KB];
+#pragma unroll
+        for (int T = 0; T < 2; ++T) {
+            const int qj = 32 * T + m;
+            const bool qv = qj < item.nq;
+            const int qc = qv ? p.q_col[item.q0 + qj] : 0;
+            if constexpr (PREC == SCREEN_BF16) {
+                const uint16_t* rh = reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)qc * KP;
+                const uint16_t* rl = reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)qc * KP;
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    short8 vh, vl;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int d = 16 * kb + 8 * h + j;
+                        const int sd = qsrc(d, p.D);
+                        float fh = qv ? bf2f(rh[sd]) : 0.0f;
+                        float fl = qv ? bf2f(rl[sd]) : 0.0f;
+                        if (d < p.D) { fh *= -2.0f; fl *= -2.0f; }
+                        vh[j] = (short)(__float_as_uint(fh) >> 16);
+                        vl[j] = (short)(__float_as_uint(fl) >> 16);
+                    }
+                    bh[T][kb] = __builtin_bit_cast(bf16x8, vh);
+                    bl[T][kb] = __builtin_bit_cast(bf16x8, vl);
+                }
+            } else {
+                const float* r = reinterpret_cast<const float*>(a.hi) + (int64_t)qc * KP;
+#pragma unroll
+                for (int g = 0; g < 2 * KB; ++g) {
+                    float v[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int d = 8 * g + 4 * h + t;
+                        float f = qv ? r[qsrc(d, p.D)] : 0.0f;
+                        if (d < p.D) f *= -2.0f;
+                        v[t] = f;
+                    }
+                    bq[T][g] = make_float4(v[0], v[1], v[2], v[3]);
+                }
+            }
+        }
+        // after the lane swap, lane l owns query l of the item
+        const bool qmine = lane < item.nq;
+        const int q_mine = item.q0 + (qmine ? lane : 0);
+        const int qc_mine = p.q_col[q_mine];
+        const float nq_mine = a.norm[qc_mine];
+
+        uint32_t L[LS];
+#pragma unroll
+        for (int i = 0; i < LS; ++i) L[i] = 0xffffffffu;
+        uint32_t tau = qmine ? 0xffffffffu : 0u;
+        int cnt = 0;
+        float nmax = 0.0f;
+        const int ib = ceil_log2(ncand > 2 ? ncand : 2);
+        const uint32_t pmask = (1u << ib) - 1u, hmask = ~pmask;
+        int run = 0;
+
+        auto compact = [&]() {
+#pragma unroll
+            for (int h0 = 0; h0 < 64; h0 += 32) {
+                if (!__any(cnt > h0)) break;
+                uint32_t B[32];
+#pragma unroll
+                for (int i = 0; i < 32; ++i) B[i] = (h0 + i < cnt) ? sbuf[(h0 + i) * 64 + lane] : 0xffffffffu;
+                bitonic_sort<32>(B);
+#pragma unroll
+                for (int i = 0; i < 32; ++i) {
+                    const uint32_t v = B[31 - i];
+                    L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
+                }
+                bitonic_clean<LS>(L);
+#pragma unroll
+                for (int i = 0; i < LS; ++i)
+                    if (i >= kprime) L[i] = 0xffffffffu;
+            }
+            cnt = 0;
+            uint32_t t = L[LS - 1];
+            if (LS == 64 && kprime == 48) t = L[47];
+            tau = qmine ? t : 0u;
+        };
+
+        for (int base = 0; base < ncand; base += 32) {
+            const int pos = base + m;
+            const bool cv = pos < ncand;
+            while (run + 1 < nr && pos >= s_ro[run + 1]) ++run;
+            const int col = cv ? s_rb[run] + (pos - s_ro[run]) : s_rb[0];
+            nmax = fmaxf(nmax, cv ? a.norm[col] : 0.0f);
+            f32x16 acc0 = {}, acc1 = {};
+            if constexpr (PREC == SCREEN_BF16) {
+                const uint16_t* rh = reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)col * KP + 8 * h;
+                const uint16_t* rl = reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)col * KP + 8 * h;
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    const bf16x8 ah = *reinterpret_cast<const bf16x8*>(rh + 16 * kb);
+                    const bf16x8 al = *reinterpret_cast<const bf16x8*>(rl + 16 * kb);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[0][kb], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[1][kb], acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[0][kb], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[1][kb], acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[0][kb], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[1][kb], acc1, 0, 0, 0);
+                }
+            } else {
+                const float* r = reinterpret_cast<const float*>(a.hi) + (int64_t)col * KP + 4 * h;
+#pragma unroll
+                for (int g = 0; g < 2 * KB; ++g) {
+                    const float4 av = *reinterpret_cast<const float4*>(r + 8 * g);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[0][g].x, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[1][g].x, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[0][g].y, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[1][g].y, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[0][g].z, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[1][g].z, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[0][g].w, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
+                }
+            }
+            // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
+            float X[16], Y[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
+                                                                false, false);
+                X[r] = __uint_as_float(sw[0]);
+                Y[r] = __uint_as_float(sw[1]);
+            }
+            const bool full = base + 32 <= ncand;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const float v = hh ? Y[r] : X[r];
+                    const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
+                    const uint32_t key = (__float_as_uint(fmaxf(v, 0.0f)) & hmask) | (uint32_t)cpos;
+                    const bool ok = key < tau && (full || cpos < ncand);
+                    if (ok) {
+                        sbuf[cnt * 64 + lane] = key;
+                        ++cnt;
+                    }
+                }
+            }
+            if (__any(cnt > 32)) compact();
+        }
+        if (__any(cnt > 0)) compact();
+
+        // ---- finalize: certification, FP64 re-rank of the ambiguous prefix, output
+        float nmax_w = nmax;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) nmax_w = fmaxf(nmax_w, __shfl_xor(nmax_w, off, 64));
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < LS; ++i) sbuf[i * 64 + lane] = L[i];
+        __syncthreads();
+        if (qmine) {
+            const int k = p.k;
+            const int nl = ncand < kprime ? ncand : kprime;
+            const int kk = ncand < k ? ncand : k;
+            const double eps = a.eta * ((double)nq_mine + (double)nmax_w) + 1e-30;
+            const double qf = ldexp(1.0, -(22 - ib));
+            const double dkt = key_dist(sbuf[(kk - 1) * 64 + lane], hmask);
+            const double U = dkt * (1.0 + qf) + eps;
+            bool ok = ncand <= kprime || ((double)key_dist(sbuf[(kprime - 1) * 64 + lane], hmask) - eps > U);
+            int r = kk;
+            while (r < nl && (double)key_dist(sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
+            const float* qrow = p.feat + (int64_t)qc_mine * p.DP;
+            double wt[WIN];
+            int wk[WIN], wc[WIN];
+            int wn = 0, out = 0;
+            double last_t = 0.0;
+            int last_c = -1;
+            for (int j = 0; j < r && ok && out < kk; ++j) {
+                const uint32_t key = sbuf[j * 64 + lane];
+                const int pos = (int)(key & pmask);
+                int lo = 0, hi = nr - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_ro[mid] <= pos) lo = mid; else hi = mid - 1;
+                }
+                const int col = s_rb[lo] + (pos - s_ro[lo]);
+                const double t = exact_d2(qrow, p.feat + (int64_t)col * p.DP, p.DP);
+                const int kc = p.col_key[col];
+                if (wn == WIN) { ok = false; break; }
+                // insert (t, kc, col) into the sorted window
+                bool placed = false;
+#pragma unroll
+                for (int i = WIN - 1; i >= 0; --i) {
+                    if (i <= wn && !placed) {
+                        if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
+                            wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
+                        } else {
+                            wt[i] = t; wk[i] = kc; wc[i] = col;
+                            placed = true;
+                        }
+                    }
+                }
+                ++wn;
+                const double lb = j + 1 < r ? (double)key_dist(sbuf[(j + 1) * 64 + lane], hmask) - eps
+                                            : __builtin_inf();
+                while (wn > 0 && out < kk && wt[0] < lb) {
+                    a.knn_col[(int64_t)q_mine * k + out] = wc[0];
+                    a.knn_t[(int64_t)q_mine * k + out] = wt[0];
+                    last_t = wt[0];
+                    last_c = wc[0];
+                    ++out;
+#pragma unroll
+                    for (int i = 0; i < WIN - 1; ++i) {
+                        if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
+                    }
+                    --wn;
+                }
+            }
+            if (ok && out == kk) {
+                for (int j = kk; j < k; ++j) {
+                    a.knn_col[(int64_t)q_mine * k + j] = last_c;
+                    a.knn_t[(int64_t)q_mine * k + j] = last_t;
+                }
+                a.dk2[q_mine] = last_t;
+            } else {
+                const int f = atomicAdd(a.n_fallback, 1);
+                a.fallback[f] = q_mine;
+            }
+        }
+    }
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Lane layout probe of the two MFMA forms and the lane swap the screen relies on.
+// out[0..1023]   : 32x32x16 bf16 with A[m][k] = (k == 0) * (m + 1), B[k][n] = (k == 0) * 64 * (n + 1)
+//                  -> acc[r] of lane l, stored at out[r * 64 + l]
+// out[1024..2047]: 32x32x2 f32 with the same A, B
+// out[2048..4095]: permlane32_swap(acc0, acc1) of the bf16 result with acc1 = acc0 + 0.5
+//                  (first and second result at out[2048 + r*64 + l], out[3072 + r*64 + l])
+__global__ void k_mfma_probe(float* out) {
+    const int l = threadIdx.x;
+    const int h = l >> 5, m = l & 31;
+    short8 va, vb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = 8 * h + j;
+        va[j] = (short)(__float_as_uint(k == 0 ? (float)(m + 1) : 0.0f) >> 16);
+        vb[j] = (short)(__float_as_uint(k == 0 ? 64.0f * (m + 1) : 0.0f) >> 16);
+    }
+    f32x16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, va), __builtin_bit_cast(bf16x8, vb), acc,
+                                                  0, 0, 0);
+    f32x16 acc2 = {};
+    acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(h == 0 ? (float)(m + 1) : 0.0f, h == 0 ? 64.0f * (m + 1) : 0.0f,
+                                                acc2, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        out[r * 64 + l] = acc[r];
+        out[1024 + r * 64 + l] = acc2[r];
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc[r]), __float_as_uint(acc[r] + 0.5f),
+                                                        false, false);
+        out[2048 + r * 64 + l] = __uint_as_float(sw[0]);
+        out[3072 + r * 64 + l] = __uint_as_float(sw[1]);
+    }
+}
+
+template <int PREC, int KB>
+hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
+    if (a.kprime <= 32)
+        k_screen_knn<PREC, KB, 32><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+    else
+        k_screen_knn<PREC, KB, 64><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int screen_kp(int D) { return (D + 2 + 15) / 16 * 16; }
+
+double screen_eta(int prec, int D, int KP) {
+    // unit roundoff taken as 2^-23 (twice FP32's) so the bound also covers truncating accumulation
+    const double u = ldexp(1.0, -23);
+    auto gamma = [&](double n) { return n * u / (1.0 - n * u); };
+    const double gD = gamma(D);
+    double eta;
+    if (prec == SCREEN_FP32) {
+        eta = 2.0 * (1.0 + gD) * gamma(KP + 2) + gD;
+    } else {
+        eta = (3.02 * ldexp(1.0, -16) + 1.02 * gamma(3 * KP + 2)) * 2.0 * (1.0 + gD) + gD;
+    }
+    return 1.25 * eta + 1e-12;
+}
+
+hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
+                              void* hi, void* lo, float* norm) {
+    if (n_cols <= 0) return hipSuccess;
+    k_screen_pack<<<(unsigned)((n_cols + 127) / 128), 128, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,
+                               const int* seg_beg) {
+    if (a.n_items <= 0) return hipSuccess;
+    k_zero_ints<<<1, 64, 0, st>>>(seg_ctr, 8);
+    const int KB = KP / 16;
+    if (prec == SCREEN_BF16) {
+        switch (KB) {
+            case 1: return launch_kb<SCREEN_BF16, 1>(st, a, grid, seg_ctr, seg_beg);
+            case 2: return launch_kb<SCREEN_BF16, 2>(st, a, grid, seg_ctr, seg_beg);
+            case 3: return launch_kb<SCREEN_BF16, 3>(st, a, grid, seg_ctr, seg_beg);
+            case 4: return launch_kb<SCREEN_BF16, 4>(st, a, grid, seg_ctr, seg_beg);
+        }
+    } else {
+        switch (KB) {
+            case 1: return launch_kb<SCREEN_FP32, 1>(st, a, grid, seg_ctr, seg_beg);
+            case 2: return launch_kb<SCREEN_FP32, 2>(st, a, grid, seg_ctr, seg_beg);
+            case 3: return launch_kb<SCREEN_FP32, 3>(st, a, grid, seg_ctr, seg_beg);
+            case 4: return launch_kb<SCREEN_FP32, 4>(st, a, grid, seg_ctr, seg_beg);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_mfma_layout_probe(hipStream_t st, float* out) {
+    k_mfma_probe<<<1, 64, 0, st>>>(out);
+    return hipGetLastError();
 }
 
 }  // namespace irc

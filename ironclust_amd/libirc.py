@@ -18,7 +18,7 @@ MEM_HOST, MEM_DEVICE = 0, 1
 EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
-    "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc",
+    "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -55,7 +55,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("n_queries", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n_screen_fallback", ctypes.c_int64), ("n_delta_full", ctypes.c_int64),
                 ("ms_total", ctypes.c_double), ("ms_knn", ctypes.c_double), ("ms_delta", ctypes.c_double),
-                ("precision_used", ctypes.c_int32)]
+                ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32)]
 
 
 class IrcError(RuntimeError):
@@ -99,6 +99,8 @@ def load(path=None):
     lib.irc_rho_delta_batch.restype = ctypes.c_int
     lib.irc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.irc_get_stats.restype = ctypes.c_int
+    lib.irc_selftest_mfma.argtypes = [ctypes.c_void_p, _c_f32p]
+    lib.irc_selftest_mfma.restype = ctypes.c_int
     lib.irc_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_f32p,
                                ctypes.c_int32, _c_f32p, _c_i32p]
     lib.irc_knn_ab.restype = ctypes.c_int
@@ -170,6 +172,11 @@ class Context:
         s = Stats()
         self._check(self.lib.irc_get_stats(self.h, ctypes.byref(s)))
         return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def selftest_mfma(self):
+        out = np.zeros(4096, np.float32)
+        self._check(self.lib.irc_selftest_mfma(self.h, _p(out, _c_f32p)))
+        return out
 
     # ---- per-site forms (search_knn_drift_ / search_delta_drift_) --------------
     def _site_problem(self, F, drift_of_col, link, in_begin, n_in, k, id_of_col):

@@ -194,3 +194,23 @@ def test_fp64_sqrt_div_rounding_matches_host(ctx):
     K_g, _ = ctx.knn_ab(B, A, 1)
     K_r, _ = O.knn_ab(B, A, 1)
     np.testing.assert_array_equal(K_g, K_r)
+
+
+def test_mfma_lane_layout(ctx):
+    """The screen relies on the 32x32 MFMA output layout (register r of lane l holds row
+    8*(r//4) + 4*(l//32) + r%4, column l%32) and on v_permlane32_swap exchanging the upper
+    half of its first operand with the lower half of its second."""
+    out = ctx.selftest_mfma().reshape(4, 16, 64)
+    r = np.arange(16)[:, None]
+    l = np.arange(64)[None, :]
+    row = 8 * (r // 4) + 4 * (l // 32) + r % 4
+    col = l % 32
+    want = (row + 1) * 64.0 * (col + 1)
+    np.testing.assert_array_equal(out[0], want, err_msg="bf16 32x32x16 layout")
+    np.testing.assert_array_equal(out[1], want, err_msg="f32 32x32x2 layout")
+    acc0 = want
+    acc1 = want + 0.5
+    first = np.where(l < 32, acc0, np.roll(acc1, 32, axis=1))
+    second = np.where(l < 32, np.roll(acc0, -32, axis=1), acc1)
+    np.testing.assert_array_equal(out[2], first, err_msg="permlane32_swap first result")
+    np.testing.assert_array_equal(out[3], second, err_msg="permlane32_swap second result")
