@@ -139,7 +139,14 @@ def cpu_baseline(ds, a):
     lib = O.cpu_port()
     cores = int(lib.cpu_max_threads())
     S = ds["n_sites"]
+    # probe one site per core, then size the sample to about 15 s of CPU work
     n_sites = a.cpu_sites or max(1, min(S, cores))
+    if not a.cpu_sites:
+        sites = np.arange(1, n_sites + 1, dtype=np.int32)
+        t = time.perf_counter()
+        O.cpu_sort_batch(host, sites=sites, nthreads=cores)
+        probe = time.perf_counter() - t
+        n_sites = int(min(S, max(n_sites, n_sites * 15.0 / max(probe, 1e-3))))
     sites = np.arange(1, n_sites + 1, dtype=np.int32)
     n_spk = int(np.isin(host["site1"], sites).sum())
     t = time.perf_counter()

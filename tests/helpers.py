@@ -71,3 +71,27 @@ def int_dataset(N, S, C, D, k, seed, lo=-3, hi=4, f2=False, link=None, lim=None)
         ds["site2"] = s2
         ds["F2"] = rng.integers(lo, hi, size=(N, D)).astype(np.float32)
     return ds
+
+
+def layout_to_ds(z):
+    """Spike-level arrays (F1, site1, F2, site2) back from the site-grouped batch layout."""
+    N = int(z["n_spikes"])
+    F, spk = np.asarray(z["F"]), np.asarray(z["col_spike"]).astype(np.int64)
+    sp, npri = np.asarray(z["site_ptr"]), np.asarray(z["site_n_primary"])
+    S = len(npri)
+    D = F.shape[1]
+    F1 = np.zeros((N, D), np.float32)
+    F2 = np.zeros((N, D), np.float32)
+    s1 = np.zeros(N, np.int32)
+    s2 = np.zeros(N, np.int32)
+    for s in range(S):
+        a, b, p = sp[s], sp[s + 1], sp[s] + npri[s]
+        F1[spk[a:p] - 1] = F[a:p]
+        s1[spk[a:p] - 1] = s + 1
+        F2[spk[p:b] - 1] = F[p:b]
+        s2[spk[p:b] - 1] = s + 1
+    has2 = bool((s2 > 0).any())
+    lim = np.asarray(z["lim"])
+    return dict(n_spikes=N, n_sites=S, D=D, n_drift=len(lim) - 1, k=int(z["k"]), F1=F1,
+                F2=F2 if has2 else None, site1=s1, site2=s2 if has2 else None, lim=lim,
+                link=np.asarray(z["link"]))

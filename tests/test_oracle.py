@@ -232,3 +232,21 @@ def test_cpu_port_agrees_with_oracle():
     assert np.mean(c["nneigh"] == r["nneigh"]) > 0.995
     ok = c["nneigh"] == r["nneigh"]
     np.testing.assert_allclose(c["delta"][ok], r["delta"][ok], rtol=1e-3)
+
+
+def test_golden_fixtures_reproduce():
+    import glob
+    import os
+    from tests.helpers import layout_to_ds
+    files = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+    assert len(files) >= 4
+    for f in files:
+        z = np.load(f)
+        ds = layout_to_ds(z)
+        r = O.sort_batch(ds)
+        for key in ("knn", "rho", "delta", "nneigh"):
+            np.testing.assert_array_equal(r[key], z[key], err_msg=f"{f} {key}")
+        if "int_ties" in f:
+            ref = brute_batch(ds)
+            for key in ("knn", "rho", "delta", "nneigh"):
+                np.testing.assert_array_equal(ref[key], z[key], err_msg=f"{f} {key} brute")
