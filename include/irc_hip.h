@@ -62,7 +62,8 @@ typedef enum {
 
 typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
-    int32_t k_over;     /* over-fetched candidates per query for the screen (0 = default 32) */
+    int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 48 or 64
+                           (0 = 64 for the BF16 form, 48 for the FP32 form) */
     int32_t tile;       /* reserved for the tuner (0 = tuned default) */
     int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
     int32_t mem;        /* irc_mem for irc_rho_delta_batch */

@@ -323,7 +323,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     }
     ctx->stats.precision_used = prec;
     const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : irc::SCREEN_BF16;
-    int kprime = k_over <= 0 ? 32 : k_over;
+    // over-fetch: the BF16 form's looser bound needs a deeper list to certify the k-th
+    int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_BF16 ? 64 : 48);
     if (kprime < p.k) kprime = p.k;
     kprime = kprime <= 32 ? 32 : (kprime <= 48 ? 48 : 64);
     // items (<= 64 queries each) in site-major order; queries outside the screen's limits go exact
@@ -361,6 +362,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
     CHK(ensure(ctx, ctx->b_scr[0], eb * (size_t)n_cols * KP));
     CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
+    CHK(ensure(ctx, ctx->b_scr[6], eb * (size_t)n_cols * KP));
+    CHK(ensure(ctx, ctx->b_scr[7], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
     CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols));
     CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
     CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
@@ -377,11 +380,13 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     HIPCHK(hipMemcpyAsync(nfb, &n_exact, sizeof(int), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_seg_beg, seg_beg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
     HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
-                                   ptr<float>(ctx->b_scr[2])));
+                                   ctx->b_scr[6].p, ctx->b_scr[7].p, ptr<float>(ctx->b_scr[2])));
     irc::ScreenArgs a;
     a.p = p;
     a.hi = ctx->b_scr[0].p;
     a.lo = ctx->b_scr[1].p;
+    a.qhi = ctx->b_scr[6].p;
+    a.qlo = ctx->b_scr[7].p;
     a.norm = ptr<float>(ctx->b_scr[2]);
     a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
     a.n_items = (int)items.size();
@@ -413,7 +418,7 @@ void irc_options_default(irc_options* opt) {
     if (!opt) return;
     memset(opt, 0, sizeof(*opt));
     opt->precision = IRC_PREC_AUTO;
-    opt->k_over = 32;
+    opt->k_over = 0;
     opt->mem = IRC_MEM_HOST;
 }
 
@@ -472,10 +477,10 @@ const char* irc_last_error(const irc_ctx* ctx) { return ctx ? ctx->err.c_str() :
 irc_status irc_selftest_mfma(irc_ctx* ctx, float* out) {
     if (!ctx || !out) return IRC_E_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    CHK(ensure(ctx, ctx->b_scr[7], sizeof(float) * 4096));
+    CHK(ensure(ctx, ctx->b_ncand, sizeof(float) * 4096));
     hipStream_t st = ctx->stream;
-    HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->b_scr[7])));
-    HIPCHK(hipMemcpyAsync(out, ctx->b_scr[7].p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
+    HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->b_ncand)));
+    HIPCHK(hipMemcpyAsync(out, ctx->b_ncand.p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -518,7 +523,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     }
     block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, p.feat, p.DP, h.n_cols, ranges,
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, p.feat, p.DP, h.n_cols, ranges,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
@@ -717,7 +722,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         }
     block_ranges_cands(C, runl, lptr, lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 32, dF, D, n_cols, ranges,
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, dF, D, n_cols, ranges,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk, o_knn,
                                   nullptr));

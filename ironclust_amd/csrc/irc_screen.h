@@ -20,6 +20,8 @@ struct ScreenArgs {
     DevPlan p;
     const void* hi;          // [n_cols][KP] bf16 bits (bf16 form) or fp32 rows (fp32 form)
     const void* lo;          // [n_cols][KP] bf16 low parts (bf16 form)
+    const void* qhi;         // [n_cols][KP] query-form rows [-2x, |x|^2, 1, 0...] (hi parts or fp32)
+    const void* qlo;         // [n_cols][KP] query-form low parts (bf16 form)
     const float* norm;       // [n_cols] fp32 |x|^2
     const ScreenItem* items;
     int n_items;
@@ -37,7 +39,7 @@ enum { SCREEN_FP32 = 0, SCREEN_BF16 = 1 };
 int screen_kp(int D);                                  // padded operand width (multiple of 16)
 double screen_eta(int prec, int D, int KP);            // rigorous error factor
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, float* norm);
+                              void* hi, void* lo, void* qhi, void* qlo, float* norm);
 // seg_ctr: 8 device counters (zeroed by the launch); seg_beg: 9 device offsets of the
 // per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,

@@ -47,30 +47,33 @@ __device__ __forceinline__ float cand_form(const float* __restrict__ F, int D, i
 }
 
 __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
-                              void* __restrict__ hi, void* __restrict__ lo, float* __restrict__ norm) {
+                              void* __restrict__ hi, void* __restrict__ lo, void* __restrict__ qhi,
+                              void* __restrict__ qlo, float* __restrict__ norm) {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= n_cols) return;
     const float* x = F + col * FS;
     float n = 0.0f;
     for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
     norm[col] = n;
-    if (prec == SCREEN_FP32) {
-        float* r = reinterpret_cast<float*>(hi) + col * KP;
-        for (int d = 0; d < KP; ++d) r[d] = cand_form(x, D, d, n);
-    } else {
-        uint16_t* rh = reinterpret_cast<uint16_t*>(hi) + col * KP;
-        uint16_t* rl = reinterpret_cast<uint16_t*>(lo) + col * KP;
-        for (int d = 0; d < KP; ++d) {
-            const float v = cand_form(x, D, d, n);
+    for (int d = 0; d < KP; ++d) {
+        const float v = cand_form(x, D, d, n);
+        // query form [-2x, |x|^2, 1, 0...]; -2 scales the hi and lo parts exactly
+        const float w = d < D ? x[d] : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
+        const float ws = d < D ? -2.0f : 1.0f;
+        if (prec == SCREEN_FP32) {
+            reinterpret_cast<float*>(hi)[col * KP + d] = v;
+            reinterpret_cast<float*>(qhi)[col * KP + d] = ws * w;
+        } else {
             const uint16_t h = f2bf_rne(v);
-            rh[d] = h;
-            rl[d] = f2bf_rne(v - bf2f(h));
+            reinterpret_cast<uint16_t*>(hi)[col * KP + d] = h;
+            reinterpret_cast<uint16_t*>(lo)[col * KP + d] = f2bf_rne(v - bf2f(h));
+            const uint16_t qh = f2bf_rne(w);
+            const uint16_t ql = f2bf_rne(w - bf2f(qh));
+            reinterpret_cast<uint16_t*>(qhi)[col * KP + d] = f2bf_rne(ws * bf2f(qh));
+            reinterpret_cast<uint16_t*>(qlo)[col * KP + d] = f2bf_rne(ws * bf2f(ql));
         }
     }
 }
-
-// query-form value of dimension d from candidate-form row values: [-2x, |x|^2, 1, 0...]
-__device__ __forceinline__ int qsrc(int d, int D) { return d < D ? d : (d == D ? D + 1 : (d == D + 1 ? D : d)); }
 
 template <int N>
 __device__ __forceinline__ void cex(uint32_t& a, uint32_t& b, bool up) {
@@ -130,10 +133,10 @@ __device__ __forceinline__ double exact_d2(const float* __restrict__ q, const fl
 
 __device__ __forceinline__ float key_dist(uint32_t key, uint32_t hmask) { return __uint_as_float(key & hmask); }
 
-constexpr int WIN = 16;  // re-rank window: longest run of mutually ambiguous neighbours
+constexpr int WIN = 8;  // re-rank window: longest run of mutually ambiguous neighbours
 
 template <int PREC, int KB, int LS>
-__global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
+__global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
                                                    const int* __restrict__ seg_beg) {
     constexpr int KP = 16 * KB;
     __shared__ uint32_t sbuf[64 * 64];
@@ -186,47 +189,30 @@ __global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict
         const int nr = s_nr;
         const int ncand = item.ncand;
 
-        // ---- query operands (B) for the two 32-query column tiles
+        // ---- query operands (B) for the two 32-query column tiles (query-form rows)
         bf16x8 bh[2][KB], bl[2][KB];
         float4 bq[2][2 * KB];
 #pragma unroll
         for (int T = 0; T < 2; ++T) {
             const int qj = 32 * T + m;
             const bool qv = qj < item.nq;
-            const int qc = qv ? p.q_col[item.q0 + qj] : 0;
+            const int qc = p.q_col[item.q0 + (qv ? qj : 0)];
             if constexpr (PREC == SCREEN_BF16) {
-                const uint16_t* rh = reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)qc * KP;
-                const uint16_t* rl = reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)qc * KP;
+                const float4* rh = reinterpret_cast<const float4*>(reinterpret_cast<const uint16_t*>(a.qhi) +
+                                                                   (int64_t)qc * KP + 8 * h);
+                const float4* rl = reinterpret_cast<const float4*>(reinterpret_cast<const uint16_t*>(a.qlo) +
+                                                                   (int64_t)qc * KP + 8 * h);
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    short8 vh, vl;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int d = 16 * kb + 8 * h + j;
-                        const int sd = qsrc(d, p.D);
-                        float fh = qv ? bf2f(rh[sd]) : 0.0f;
-                        float fl = qv ? bf2f(rl[sd]) : 0.0f;
-                        if (d < p.D) { fh *= -2.0f; fl *= -2.0f; }
-                        vh[j] = (short)(__float_as_uint(fh) >> 16);
-                        vl[j] = (short)(__float_as_uint(fl) >> 16);
-                    }
-                    bh[T][kb] = __builtin_bit_cast(bf16x8, vh);
-                    bl[T][kb] = __builtin_bit_cast(bf16x8, vl);
+                    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                    bh[T][kb] = __builtin_bit_cast(bf16x8, qv ? rh[2 * kb] : z);
+                    bl[T][kb] = __builtin_bit_cast(bf16x8, qv ? rl[2 * kb] : z);
                 }
             } else {
-                const float* r = reinterpret_cast<const float*>(a.hi) + (int64_t)qc * KP;
+                const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(a.qhi) +
+                                                                  (int64_t)qc * KP + 4 * h);
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) {
-                    float v[4];
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) {
-                        const int d = 8 * g + 4 * h + t;
-                        float f = qv ? r[qsrc(d, p.D)] : 0.0f;
-                        if (d < p.D) f *= -2.0f;
-                        v[t] = f;
-                    }
-                    bq[T][g] = make_float4(v[0], v[1], v[2], v[3]);
-                }
+                for (int g = 0; g < 2 * KB; ++g) bq[T][g] = qv ? r[2 * g] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
         // after the lane swap, lane l owns query l of the item
@@ -259,9 +245,10 @@ __global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict
                     L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
                 }
                 bitonic_clean<LS>(L);
+                if (LS == 64 && kprime == 48) {
 #pragma unroll
-                for (int i = 0; i < LS; ++i)
-                    if (i >= kprime) L[i] = 0xffffffffu;
+                    for (int i = 48; i < LS; ++i) L[i] = 0xffffffffu;
+                }
             }
             cnt = 0;
             uint32_t t = L[LS - 1];
@@ -269,20 +256,41 @@ __global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict
             tau = qmine ? t : 0u;
         };
 
-        for (int base = 0; base < ncand; base += 32) {
-            const int pos = base + m;
-            const bool cv = pos < ncand;
+        // operand rows of the lane's candidate at stream position pos (bf16: hi then lo halves)
+        constexpr int NA = PREC == SCREEN_BF16 ? 2 * KB : 2 * KB;
+        float4 An[NA];
+        float nnN = 0.0f;
+        auto issue = [&](int pos) {
             while (run + 1 < nr && pos >= s_ro[run + 1]) ++run;
-            const int col = cv ? s_rb[run] + (pos - s_ro[run]) : s_rb[0];
-            nmax = fmaxf(nmax, cv ? a.norm[col] : 0.0f);
-            f32x16 acc0 = {}, acc1 = {};
+            const int col = pos < ncand ? s_rb[run] + (pos - s_ro[run]) : s_rb[0];
+            nnN = pos < ncand ? a.norm[col] : 0.0f;
             if constexpr (PREC == SCREEN_BF16) {
-                const uint16_t* rh = reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)col * KP + 8 * h;
-                const uint16_t* rl = reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)col * KP + 8 * h;
+                const float4* rh = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)col * KP + 8 * h);
+                const float4* rl = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)col * KP + 8 * h);
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    const bf16x8 ah = *reinterpret_cast<const bf16x8*>(rh + 16 * kb);
-                    const bf16x8 al = *reinterpret_cast<const bf16x8*>(rl + 16 * kb);
+                    An[2 * kb] = rh[2 * kb];
+                    An[2 * kb + 1] = rl[2 * kb];
+                }
+            } else {
+                const float4* r = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const float*>(a.hi) + (int64_t)col * KP + 4 * h);
+#pragma unroll
+                for (int g = 0; g < 2 * KB; ++g) An[g] = r[2 * g];
+            }
+        };
+        issue(m);
+        for (int base = 0; base < ncand; base += 32) {
+            nmax = fmaxf(nmax, nnN);
+            const float4* Ac = An;
+            f32x16 acc0 = {}, acc1 = {};
+            if constexpr (PREC == SCREEN_BF16) {
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
+                    const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
                     acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[0][kb], acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[1][kb], acc1, 0, 0, 0);
                     acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[0][kb], acc0, 0, 0, 0);
@@ -291,10 +299,9 @@ __global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[1][kb], acc1, 0, 0, 0);
                 }
             } else {
-                const float* r = reinterpret_cast<const float*>(a.hi) + (int64_t)col * KP + 4 * h;
 #pragma unroll
                 for (int g = 0; g < 2 * KB; ++g) {
-                    const float4 av = *reinterpret_cast<const float4*>(r + 8 * g);
+                    const float4 av = Ac[g];
                     acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[0][g].x, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[1][g].x, acc1, 0, 0, 0);
                     acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[0][g].y, acc0, 0, 0, 0);
@@ -305,23 +312,22 @@ __global__ __launch_bounds__(64) void k_screen_knn(ScreenArgs a, int* __restrict
                     acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
                 }
             }
-            // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
-            float X[16], Y[16];
+            // the operand registers are free once the MFMAs have read them: the next tile's
+            // loads are in flight while this tile's distances are filtered
+            if (base + 32 < ncand) issue(base + 32 + m);
+            // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1.  Keys:
+            // non-negative distance bits (a negative screened value counts as 0) above the
+            // candidate's stream position.
+            const bool full = base + 32 <= ncand;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
                                                                 false, false);
-                X[r] = __uint_as_float(sw[0]);
-                Y[r] = __uint_as_float(sw[1]);
-            }
-            const bool full = base + 32 <= ncand;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
 #pragma unroll
                 for (int hh = 0; hh < 2; ++hh) {
-                    const float v = hh ? Y[r] : X[r];
+                    const int bits = max((int)sw[hh], 0);
                     const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
-                    const uint32_t key = (__float_as_uint(fmaxf(v, 0.0f)) & hmask) | (uint32_t)cpos;
+                    const uint32_t key = ((uint32_t)bits & hmask) | (uint32_t)cpos;
                     const bool ok = key < tau && (full || cpos < ncand);
                     if (ok) {
                         sbuf[cnt * 64 + lane] = key;
@@ -479,9 +485,10 @@ double screen_eta(int prec, int D, int KP) {
 }
 
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, float* norm) {
+                              void* hi, void* lo, void* qhi, void* qlo, float* norm) {
     if (n_cols <= 0) return hipSuccess;
-    k_screen_pack<<<(unsigned)((n_cols + 127) / 128), 128, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, norm);
+    k_screen_pack<<<(unsigned)((n_cols + 127) / 128), 128, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
+                                                                     norm);
     return hipGetLastError();
 }
 

@@ -38,7 +38,7 @@ def _opts(P):
     if isinstance(prec, str):
         prec = {"exact": libirc.PREC_EXACT, "fp32": libirc.PREC_FP32, "bf16": libirc.PREC_BF16_SCREEN,
                 "auto": libirc.PREC_AUTO}[prec]
-    return libirc.make_options(precision=prec, k_over=int(P.get("k_over", 32)),
+    return libirc.make_options(precision=prec, k_over=int(P.get("k_over", 0)),
                                force_fail=bool(P.get("force_fail", False)), flags=int(P.get("flags", 0)))
 
 

@@ -130,7 +130,7 @@ def _dptr(t, ct):
     return ctypes.cast(ctypes.c_void_p(t.data_ptr()), ct)
 
 
-def make_options(precision=PREC_AUTO, k_over=32, force_fail=False, mem=MEM_HOST, flags=0):
+def make_options(precision=PREC_AUTO, k_over=0, force_fail=False, mem=MEM_HOST, flags=0):
     lib = load()
     o = Options()
     lib.irc_options_default(ctypes.byref(o))

@@ -37,7 +37,7 @@ def test_library_is_gfx950_code_object():
 
 def test_options_default():
     o = libirc.make_options()
-    assert o.precision == libirc.PREC_AUTO and o.force_fail == 0 and o.k_over == 32
+    assert o.precision == libirc.PREC_AUTO and o.force_fail == 0 and o.k_over == 0
 
 
 def test_ctx_create_rejects_bad_device():
