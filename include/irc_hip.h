@@ -64,7 +64,8 @@ typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
     int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 48 or 64
                            (0 = 64 for the BF16 form, 48 for the FP32 form) */
-    int32_t tile;       /* reserved for the tuner (0 = tuned default) */
+    int32_t tile;       /* smallest query group screened per wavefront; smaller groups (the
+                           leftovers of a block) use the exact kernel (0 = 8, < 0 = never) */
     int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
     int32_t mem;        /* irc_mem for irc_rho_delta_batch */
     int32_t flags;      /* bit 0: per-(site,chunk) launches instead of the batched launch */

@@ -274,6 +274,12 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// query groups smaller than this go to the exact kernel (irc_options.tile; 0 = 8, < 0 = never)
+int min_item(const irc_options* opt) {
+    const int t = opt ? opt->tile : 0;
+    return t == 0 ? 8 : (t < 0 ? 0 : t);
+}
+
 int resolve_precision(const irc_options* opt) {
     int prec = opt ? opt->precision : IRC_PREC_AUTO;
     if (prec == IRC_PREC_AUTO) prec = IRC_PREC_BF16_SCREEN;
@@ -305,8 +311,9 @@ void block_ranges_cands(int C, const std::vector<int>& runl, const std::vector<i
 
 // kNN pass: MFMA screen + certification + FP64 re-rank for every block that fits the
 // screen, the exact FP64 kernel for the rest and for uncertified queries.
-irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, const float* F, int FS,
-                   int64_t n_cols, const std::vector<QRange>& ranges, int* knn_col, double* knn_t, double* dk2) {
+irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
+                   int FS, int64_t n_cols, const std::vector<QRange>& ranges, int* knn_col, double* knn_t,
+                   double* dk2) {
     hipStream_t st = ctx->stream;
     int64_t pairs = 0;
     for (const auto& r : ranges) pairs += (int64_t)r.nq * r.ncand;
@@ -339,6 +346,12 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
         }
         for (int i = 0; i < r.nq; i += 64) {
             const int n = r.nq - i < 64 ? r.nq - i : 64;
+            if (n < min_item) {
+                // a wavefront streams the whole candidate set whatever its query count: a few
+                // leftover queries cost less on the exact kernel
+                for (int j = 0; j < n; ++j) exact_q.push_back(r.q0 + i + j);
+                continue;
+            }
             items.push_back({r.blk, r.q0 + i, n, r.ncand});
             cost.push_back((double)r.ncand + 64.0);
         }
@@ -523,7 +536,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     }
     block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, p.feat, p.DP, h.n_cols, ranges,
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols, ranges,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
@@ -722,7 +735,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         }
     block_ranges_cands(C, runl, lptr, lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, dF, D, n_cols, ranges,
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, ranges,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk, o_knn,
                                   nullptr));
@@ -774,7 +787,7 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
-    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, p.feat, p.DP, h.n_cols, {}, ptr<int>(ctx->b_knncol),
+    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, {}, ptr<int>(ctx->b_knncol),
                 ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));

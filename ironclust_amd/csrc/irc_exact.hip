@@ -139,18 +139,14 @@ __global__ void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict_
                              float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
                              int* __restrict__ knn32) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nq) {
-        const int slot = p.q_out[i];
-        const double d = dk2[i];
-        if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
-        if (dk_slot) dk_slot[slot] = (float)sqrt(d);
-    }
-    const int64_t total = (int64_t)nq * p.k;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t qi = e / p.k;
-        const int j = (int)(e - qi * p.k);
-        const int slot = p.q_out[qi];
-        const int v = p.col_ret[knn_col[e]];
+    if (i >= nq) return;
+    const int slot = p.q_out[i];
+    const double d = dk2[i];
+    if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
+    if (dk_slot) dk_slot[slot] = (float)sqrt(d);
+    const int* src = knn_col + i * p.k;
+    for (int j = 0; j < p.k; ++j) {
+        const int v = p.col_ret[src[j]];
         if (knn64) knn64[(int64_t)slot * p.k + j] = v;
         if (knn32) knn32[(int64_t)slot * p.k + j] = v;
     }
@@ -389,12 +385,8 @@ hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qli
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
                              const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32) {
     if (nq <= 0) return hipSuccess;
-    const int64_t total = (int64_t)nq * p.k;
-    int64_t blocks = (total + 255) / 256;
-    if (blocks < (nq + 255) / 256) blocks = (nq + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    if (blocks < (nq + 255) / 256) blocks = (nq + 255) / 256;
-    k_knn_finish<<<(unsigned)blocks, 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot, knn64, knn32);
+    k_knn_finish<<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot, knn64,
+                                                               knn32);
     return hipGetLastError();
 }
 

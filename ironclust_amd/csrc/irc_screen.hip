@@ -41,37 +41,58 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// value of dimension d of the candidate-form row [x, 1, |x|^2, 0...]
-__device__ __forceinline__ float cand_form(const float* __restrict__ F, int D, int d, float n) {
-    return d < D ? F[d] : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
-}
-
+// One thread per (column, 8-dimension chunk): coalesced 16-byte stores of the candidate-form
+// rows [x, 1, |x|^2, 0...] and the query-form rows [-2x, |x|^2, 1, 0...] (bf16 hi/lo split, or fp32).
 __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
                               void* __restrict__ hi, void* __restrict__ lo, void* __restrict__ qhi,
                               void* __restrict__ qlo, float* __restrict__ norm) {
-    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= n_cols) return;
+    const int nch = KP / 8;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_cols * nch) return;
+    const int64_t col = e / nch;
+    const int j = (int)(e - col * nch);
     const float* x = F + col * FS;
     float n = 0.0f;
     for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
-    norm[col] = n;
-    for (int d = 0; d < KP; ++d) {
-        const float v = cand_form(x, D, d, n);
-        // query form [-2x, |x|^2, 1, 0...]; -2 scales the hi and lo parts exactly
-        const float w = d < D ? x[d] : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
-        const float ws = d < D ? -2.0f : 1.0f;
-        if (prec == SCREEN_FP32) {
-            reinterpret_cast<float*>(hi)[col * KP + d] = v;
-            reinterpret_cast<float*>(qhi)[col * KP + d] = ws * w;
-        } else {
-            const uint16_t h = f2bf_rne(v);
-            reinterpret_cast<uint16_t*>(hi)[col * KP + d] = h;
-            reinterpret_cast<uint16_t*>(lo)[col * KP + d] = f2bf_rne(v - bf2f(h));
-            const uint16_t qh = f2bf_rne(w);
-            const uint16_t ql = f2bf_rne(w - bf2f(qh));
-            reinterpret_cast<uint16_t*>(qhi)[col * KP + d] = f2bf_rne(ws * bf2f(qh));
-            reinterpret_cast<uint16_t*>(qlo)[col * KP + d] = f2bf_rne(ws * bf2f(ql));
+    if (j == 0) norm[col] = n;
+    float cv[8], qv[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int d = 8 * j + t;
+        const float xv = d < D ? x[d] : 0.0f;
+        cv[t] = d < D ? xv : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
+        qv[t] = d < D ? xv : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
+    }
+    const int64_t off = col * KP + 8 * j;
+    if (prec == SCREEN_FP32) {
+        float4* h4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(hi) + off);
+        float4* q4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(qhi) + off);
+        h4[0] = make_float4(cv[0], cv[1], cv[2], cv[3]);
+        h4[1] = make_float4(cv[4], cv[5], cv[6], cv[7]);
+        float s[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) s[t] = 8 * j + t < D ? -2.0f * qv[t] : qv[t];
+        q4[0] = make_float4(s[0], s[1], s[2], s[3]);
+        q4[1] = make_float4(s[4], s[5], s[6], s[7]);
+    } else {
+        uint32_t ph[4], pl[4], qh[4], ql[4];
+#pragma unroll
+        for (int t = 0; t < 8; t += 2) {
+            uint16_t h0 = f2bf_rne(cv[t]), h1 = f2bf_rne(cv[t + 1]);
+            uint16_t l0 = f2bf_rne(cv[t] - bf2f(h0)), l1 = f2bf_rne(cv[t + 1] - bf2f(h1));
+            ph[t / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            pl[t / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            // -2 scales the hi and lo parts of the query value exactly
+            const float w0 = 8 * j + t < D ? -2.0f : 1.0f, w1 = 8 * j + t + 1 < D ? -2.0f : 1.0f;
+            uint16_t a0 = f2bf_rne(qv[t]), a1 = f2bf_rne(qv[t + 1]);
+            uint16_t b0 = f2bf_rne(qv[t] - bf2f(a0)), b1 = f2bf_rne(qv[t + 1] - bf2f(a1));
+            qh[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(a0)) | ((uint32_t)f2bf_rne(w1 * bf2f(a1)) << 16);
+            ql[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(b0)) | ((uint32_t)f2bf_rne(w1 * bf2f(b1)) << 16);
         }
+        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(hi) + off) = make_uint4(ph[0], ph[1], ph[2], ph[3]);
+        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(lo) + off) = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(qhi) + off) = make_uint4(qh[0], qh[1], qh[2], qh[3]);
+        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(qlo) + off) = make_uint4(ql[0], ql[1], ql[2], ql[3]);
     }
 }
 
@@ -487,8 +508,9 @@ double screen_eta(int prec, int D, int KP) {
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
                               void* hi, void* lo, void* qhi, void* qlo, float* norm) {
     if (n_cols <= 0) return hipSuccess;
-    k_screen_pack<<<(unsigned)((n_cols + 127) / 128), 128, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
-                                                                     norm);
+    const int64_t total = n_cols * (KP / 8);
+    k_screen_pack<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
+                                                                    norm);
     return hipGetLastError();
 }
 
