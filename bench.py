@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--cpu-sites", type=int, default=0, help="sites in the CPU baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify size-independent properties of the output")
+    ap.add_argument("--copy-calib", type=float, default=0.0,
+                    help="only run a plain device copy of this many GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -45,6 +47,14 @@ def main():
     import torch
     from ironclust_amd import irc2, libirc, synth
 
+    if a.copy_calib > 0:
+        n = int(a.copy_calib * (1 << 30)) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n}))
+        return
     t0 = time.time()
     ds = synth.make_workload(a.workload, device="cuda")
     lay = irc2.site_layout(ds)

@@ -62,13 +62,14 @@ typedef enum {
 
 typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
-    int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 48 or 64
-                           (0 = 64 for the BF16 form, 48 for the FP32 form) */
+    int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 40, 48 or 64
+                           (0 = 48 for the BF16 form, 40 for the FP32 form) */
     int32_t tile;       /* smallest query group screened per wavefront; smaller groups (the
-                           leftovers of a block) use the exact kernel (0 = 8, < 0 = never) */
+                           leftovers of a block) use the exact kernel (0 = never) */
     int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
     int32_t mem;        /* irc_mem for irc_rho_delta_batch */
-    int32_t flags;      /* bit 0: per-(site,chunk) launches instead of the batched launch */
+    int32_t flags;      /* bit 1: run the screen's histogram pre-pass (a first pass that sets each
+                           query's starting threshold; off by default) */
 } irc_options;
 
 /* Fill *opt with defaults. */

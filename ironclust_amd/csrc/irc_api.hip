@@ -35,6 +35,7 @@ struct irc_ctx {
     hipEvent_t ev[6] = {};
     std::string err;
     irc_stats stats{};
+    int flags = 0;       // irc_options.flags of the current call
     // device buffers, grown on demand and kept across calls
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
     DevBuf b_small, b_flags, b_knncol, b_knnt, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
@@ -95,6 +96,7 @@ int pad_dim(int D) { return (D + 3) / 4 * 4; }
 
 irc_status check_opt(irc_ctx* ctx, const irc_options* opt) {
     if (opt && opt->force_fail) return fail(ctx, IRC_E_FORCED, "failure forced by irc_options.force_fail");
+    ctx->flags = opt ? opt->flags : 0;
     return IRC_OK;
 }
 
@@ -274,10 +276,10 @@ double event_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// query groups smaller than this go to the exact kernel (irc_options.tile; 0 = 8, < 0 = never)
+// query groups smaller than this go to the exact kernel (irc_options.tile; 0 or < 0 = never)
 int min_item(const irc_options* opt) {
     const int t = opt ? opt->tile : 0;
-    return t == 0 ? 8 : (t < 0 ? 0 : t);
+    return t > 0 ? t : 0;
 }
 
 int resolve_precision(const irc_options* opt) {
@@ -331,9 +333,9 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     ctx->stats.precision_used = prec;
     const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : irc::SCREEN_BF16;
     // over-fetch: the BF16 form's looser bound needs a deeper list to certify the k-th
-    int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_BF16 ? 64 : 48);
+    int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_BF16 ? 48 : 40);
     if (kprime < p.k) kprime = p.k;
-    kprime = kprime <= 32 ? 32 : (kprime <= 48 ? 48 : 64);
+    kprime = kprime <= 32 ? 32 : (kprime <= 40 ? 40 : (kprime <= 48 ? 48 : 64));
     // items (<= 64 queries each) in site-major order; queries outside the screen's limits go exact
     std::vector<irc::ScreenItem> items;
     std::vector<int> exact_q;
@@ -404,6 +406,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
     a.n_items = (int)items.size();
     a.kprime = kprime;
+    a.hist = (ctx->flags & 2) ? 1 : 0;
     a.eta = irc::screen_eta(sp, p.D, KP);
     a.knn_col = knn_col;
     a.knn_t = knn_t;

@@ -134,22 +134,24 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
 //   mode 0 (batch): rho_slot[slot] = float(1/sqrt(dk2)); dk_slot (opt) = float(sqrt(dk2));
 //                   knn64[slot*k+j] = col_ret (opt)
 //   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret
+// block (k, QPB): threadIdx.x = neighbour rank, threadIdx.y = query within the block, so the k
+// entries of a query are read and written as contiguous runs
 __global__ void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
                              const double* __restrict__ dk2, float* __restrict__ rho_slot,
                              float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
                              int* __restrict__ knn32) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
     if (i >= nq) return;
+    const int j = threadIdx.x;
     const int slot = p.q_out[i];
-    const double d = dk2[i];
-    if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
-    if (dk_slot) dk_slot[slot] = (float)sqrt(d);
-    const int* src = knn_col + i * p.k;
-    for (int j = 0; j < p.k; ++j) {
-        const int v = p.col_ret[src[j]];
-        if (knn64) knn64[(int64_t)slot * p.k + j] = v;
-        if (knn32) knn32[(int64_t)slot * p.k + j] = v;
+    if (j == 0) {
+        const double d = dk2[i];
+        if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
+        if (dk_slot) dk_slot[slot] = (float)sqrt(d);
     }
+    const int v = p.col_ret[knn_col[i * p.k + j]];
+    if (knn64) knn64[(int64_t)slot * p.k + j] = v;
+    if (knn32) knn32[(int64_t)slot * p.k + j] = v;
 }
 
 // rho per column from rho per slot (batch mode: slot = spike id - 1 = col_key - 1)
@@ -385,8 +387,10 @@ hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qli
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
                              const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32) {
     if (nq <= 0) return hipSuccess;
-    k_knn_finish<<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot, knn64,
-                                                               knn32);
+    const int qpb = 256 / p.k > 0 ? 256 / p.k : 1;
+    dim3 block(p.k, qpb);
+    k_knn_finish<<<(unsigned)((nq + qpb - 1) / qpb), block, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot,
+                                                                     knn64, knn32);
     return hipGetLastError();
 }
 

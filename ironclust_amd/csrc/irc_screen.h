@@ -26,6 +26,7 @@ struct ScreenArgs {
     const ScreenItem* items;
     int n_items;
     int kprime;              // over-fetched candidates per query (32, 48 or 64)
+    int hist;                // run the histogram pre-pass that sets each query's starting threshold
     double eta;              // |screened - exact| <= eta * (|q|^2 + |c|^2)
     int* knn_col;
     double* knn_t;

@@ -21,6 +21,8 @@
 // all 32 distances of its own query.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "irc_kernels.h"
 #include "irc_screen.h"
 
@@ -156,11 +158,11 @@ __device__ __forceinline__ float key_dist(uint32_t key, uint32_t hmask) { return
 
 constexpr int WIN = 8;  // re-rank window: longest run of mutually ambiguous neighbours
 
-template <int PREC, int KB, int LS>
+template <int PREC, int KB, int LS, bool HIST>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
                                                    const int* __restrict__ seg_beg) {
     constexpr int KP = 16 * KB;
-    __shared__ uint32_t sbuf[64 * 64];
+    __shared__ uint32_t sbuf[65 * 64];   // rows 0..63: append buffer / histogram / list; row 64: sink
     __shared__ int s_rb[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_ro[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_nr, s_item;
@@ -252,29 +254,35 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         const uint32_t pmask = (1u << ib) - 1u, hmask = ~pmask;
         int run = 0;
 
-        auto compact = [&]() {
+        // the lane's K'-th kept key: the append threshold (L may hold up to LS >= K' keys; those
+        // past K' are larger than the threshold and never change the result)
+        auto kth = [&]() -> uint32_t {
+            if constexpr (LS == 32) return L[31];
+            else return kprime == 40 ? L[39] : (kprime == 48 ? L[47] : (kprime == 32 ? L[31] : L[63]));
+        };
+        uint32_t tau_init = tau;
+        // merge the oldest 32 buffered keys into L, move the rest (at most 32) down
+        auto compact_half = [&]() {
+            uint32_t B[32];
 #pragma unroll
-            for (int h0 = 0; h0 < 64; h0 += 32) {
-                if (!__any(cnt > h0)) break;
-                uint32_t B[32];
-#pragma unroll
-                for (int i = 0; i < 32; ++i) B[i] = (h0 + i < cnt) ? sbuf[(h0 + i) * 64 + lane] : 0xffffffffu;
-                bitonic_sort<32>(B);
-#pragma unroll
-                for (int i = 0; i < 32; ++i) {
-                    const uint32_t v = B[31 - i];
-                    L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
-                }
-                bitonic_clean<LS>(L);
-                if (LS == 64 && kprime == 48) {
-#pragma unroll
-                    for (int i = 48; i < LS; ++i) L[i] = 0xffffffffu;
-                }
+            for (int i = 0; i < 32; ++i) {
+                const uint32_t v = sbuf[i * 64 + lane];
+                B[i] = i < cnt ? v : 0xffffffffu;
             }
-            cnt = 0;
-            uint32_t t = L[LS - 1];
-            if (LS == 64 && kprime == 48) t = L[47];
-            tau = qmine ? t : 0u;
+            bitonic_sort<32>(B);
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const uint32_t v = B[31 - i];
+                L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
+            }
+            bitonic_clean<LS>(L);
+            if (__any(cnt > 32)) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = sbuf[(32 + i) * 64 + lane];
+            }
+            cnt = cnt > 32 ? cnt - 32 : 0;
+            const uint32_t t = kth();
+            tau = qmine ? (t < tau_init ? t : tau_init) : 0u;
         };
 
         // operand rows of the lane's candidate at stream position pos (bf16: hi then lo halves)
@@ -302,63 +310,109 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 for (int g = 0; g < 2 * KB; ++g) An[g] = r[2 * g];
             }
         };
-        issue(m);
-        for (int base = 0; base < ncand; base += 32) {
-            nmax = fmaxf(nmax, nnN);
-            const float4* Ac = An;
-            f32x16 acc0 = {}, acc1 = {};
-            if constexpr (PREC == SCREEN_BF16) {
+        // one pass over the candidate stream: MFMA distance tiles, then consume(v, cpos, valid)
+        // for the 32 distances of the lane's query, then after_tile()
+        auto stream = [&](auto&& consume, auto&& after_tile) {
+            run = 0;
+            issue(m);
+            for (int base = 0; base < ncand; base += 32) {
+                nmax = fmaxf(nmax, nnN);
+                const float4* Ac = An;
+                f32x16 acc0 = {}, acc1 = {};
+                if constexpr (PREC == SCREEN_BF16) {
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) {
-                    const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
-                    const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[0][kb], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[1][kb], acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[0][kb], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[1][kb], acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[0][kb], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[1][kb], acc1, 0, 0, 0);
-                }
-            } else {
+                    for (int kb = 0; kb < KB; ++kb) {
+                        const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
+                        const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[0][kb], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[1][kb], acc1, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[0][kb], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[1][kb], acc1, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[0][kb], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[1][kb], acc1, 0, 0, 0);
+                    }
+                } else {
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) {
-                    const float4 av = Ac[g];
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[0][g].x, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[1][g].x, acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[0][g].y, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[1][g].y, acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[0][g].z, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[1][g].z, acc1, 0, 0, 0);
-                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[0][g].w, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
-                }
-            }
-            // the operand registers are free once the MFMAs have read them: the next tile's
-            // loads are in flight while this tile's distances are filtered
-            if (base + 32 < ncand) issue(base + 32 + m);
-            // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1.  Keys:
-            // non-negative distance bits (a negative screened value counts as 0) above the
-            // candidate's stream position.
-            const bool full = base + 32 <= ncand;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
-                                                                false, false);
-#pragma unroll
-                for (int hh = 0; hh < 2; ++hh) {
-                    const int bits = max((int)sw[hh], 0);
-                    const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
-                    const uint32_t key = ((uint32_t)bits & hmask) | (uint32_t)cpos;
-                    const bool ok = key < tau && (full || cpos < ncand);
-                    if (ok) {
-                        sbuf[cnt * 64 + lane] = key;
-                        ++cnt;
+                    for (int g = 0; g < 2 * KB; ++g) {
+                        const float4 av = Ac[g];
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[0][g].x, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[1][g].x, acc1, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[0][g].y, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[1][g].y, acc1, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[0][g].z, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[1][g].z, acc1, 0, 0, 0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[0][g].w, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
                     }
                 }
+                // the operand registers are free once the MFMAs have read them: the next tile's
+                // loads are in flight while this tile's distances are consumed
+                if (base + 32 < ncand) issue(base + 32 + m);
+                // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
+                auto tile_out = [&](auto full) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]),
+                                                                        __float_as_uint(acc1[r]), false, false);
+#pragma unroll
+                        for (int hh = 0; hh < 2; ++hh) {
+                            const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
+                            consume((int)sw[hh], cpos, decltype(full)::value || cpos < ncand);
+                        }
+                    }
+                };
+                if (base + 32 <= ncand) tile_out(std::true_type{});
+                else tile_out(std::false_type{});
+                after_tile();
             }
-            if (__any(cnt > 32)) compact();
+        };
+
+        // ---- pass 1 (HIST): per-lane histogram of the screened distances in log buckets (4 per
+        // octave, 16 octaves from |q|^2 * 2^-14) gives a starting threshold that already admits
+        // at least K' candidates, so pass 2 appends about K' keys instead of K'(1 + ln(n/K')).
+        // Row 64 of sbuf absorbs the writes of masked-out elements (no divergent branches).
+        if constexpr (HIST) {
+            if (ncand > kprime) {
+                __syncthreads();
+                uint4* z = reinterpret_cast<uint4*>(sbuf);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) z[i * 64 + lane] = make_uint4(0u, 0u, 0u, 0u);
+                __syncthreads();
+                const int hbase = (int)(__float_as_uint(fmaxf(nq_mine, 1e-30f) * 0x1p-14f) & 0xffe00000u);
+                stream(
+                    [&](int bits, int, bool valid) {
+                        // negative screened values give negative bits: clamped into bucket 0
+                        int b = (bits - hbase) >> 21;
+                        b = b < 0 ? 0 : (b > 63 ? 63 : b);
+                        atomicAdd(&sbuf[(valid ? b : 64) * 64 + lane], 1u);
+                    },
+                    [&]() {});
+                __syncthreads();
+                int cum = 0, tb = 63;
+                for (int b = 0; b < 63; ++b) {
+                    cum += (int)sbuf[b * 64 + lane];
+                    if (cum >= kprime) { tb = b; break; }
+                }
+                if (qmine && tb < 63) tau = (uint32_t)(hbase + ((tb + 1) << 21));
+                tau_init = tau;
+                __syncthreads();
+            }
         }
-        if (__any(cnt > 0)) compact();
+        nmax = 0.0f;
+
+        // ---- pass 2: keys (non-negative distance bits above the candidate's stream position)
+        // below the threshold go to the lane's append buffer; full buffers merge into L
+        stream(
+            [&](int sbits, int cpos, bool valid) {
+                const uint32_t key = ((uint32_t)max(sbits, 0) & hmask) | (uint32_t)cpos;
+                const bool ok = key < tau && valid;
+                sbuf[(ok ? cnt : 64) * 64 + lane] = key;
+                cnt += ok ? 1 : 0;
+            },
+            [&]() {
+                if (__any(cnt > 32)) compact_half();
+            });
+        while (__any(cnt > 0)) compact_half();
 
         // ---- finalize: certification, FP64 re-rank of the ambiguous prefix, output
         float nmax_w = nmax;
@@ -376,7 +430,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const double qf = ldexp(1.0, -(22 - ib));
             const double dkt = key_dist(sbuf[(kk - 1) * 64 + lane], hmask);
             const double U = dkt * (1.0 + qf) + eps;
-            bool ok = ncand <= kprime || ((double)key_dist(sbuf[(kprime - 1) * 64 + lane], hmask) - eps > U);
+            // every candidate outside L has a key at least min(L[K'-1], starting threshold)
+            const uint32_t kexcl = sbuf[(kprime - 1) * 64 + lane] < tau_init ? sbuf[(kprime - 1) * 64 + lane] : tau_init;
+            bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
             int r = kk;
             while (r < nl && (double)key_dist(sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
             const float* qrow = p.feat + (int64_t)qc_mine * p.DP;
@@ -480,10 +536,13 @@ __global__ void k_mfma_probe(float* out) {
 
 template <int PREC, int KB>
 hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
-    if (a.kprime <= 32)
-        k_screen_knn<PREC, KB, 32><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
-    else
-        k_screen_knn<PREC, KB, 64><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+    if (a.kprime <= 32) {
+        if (a.hist) k_screen_knn<PREC, KB, 32, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+        else k_screen_knn<PREC, KB, 32, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+    } else {
+        if (a.hist) k_screen_knn<PREC, KB, 64, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+        else k_screen_knn<PREC, KB, 64, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+    }
     return hipGetLastError();
 }
 
