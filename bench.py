@@ -78,7 +78,7 @@ def main():
         for _ in range(a.warmup):
             ctx.rho_delta_batch_device(dev, out, opts)
         torch.cuda.synchronize()
-        st_acc = {"ms_knn": 0.0, "ms_delta": 0.0, "ms_total": 0.0, "ms_screen": 0.0}
+        st_acc = {"ms_knn": 0.0, "ms_delta": 0.0, "ms_total": 0.0, "ms_screen": 0.0, "ms_rerank": 0.0}
         t1 = time.perf_counter()
         for _ in range(a.steps):
             ctx.rho_delta_batch_device(dev, out, opts)
@@ -94,7 +94,7 @@ def main():
             "spikes_per_s": N / dt, "ms_per_step": dt * 1e3,
             "ms_knn": st_acc["ms_knn"] / a.steps, "ms_delta": st_acc["ms_delta"] / a.steps,
             "ms_device_total": st_acc["ms_total"] / a.steps,
-            "knn_kernel_ms": ms_kernel,
+            "knn_kernel_ms": ms_kernel, "rerank_ms": st_acc["ms_rerank"] / a.steps,
             "knn_tflops": flops / (ms_kernel * 1e-3) / 1e12,
             "screen_fallback": int(s["n_screen_fallback"]), "delta_full_scan": int(s["n_delta_full"]),
             "precision_used": int(s["precision_used"]),

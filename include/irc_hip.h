@@ -180,6 +180,7 @@ typedef struct {
     double ms_delta;          /* delta kernels */
     double ms_screen;         /* the screen kernel alone (0 on the exact path) */
     int32_t precision_used;
+    double ms_rerank;         /* the FP64 re-rank of the certified lists (0 on the exact path) */
 } irc_stats;
 
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);

@@ -32,7 +32,7 @@ struct irc_ctx {
     int device = 0;
     int n_cu = 256;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};
     std::string err;
     irc_stats stats{};
     int flags = 0;       // irc_options.flags of the current call
@@ -41,6 +41,7 @@ struct irc_ctx {
     DevBuf b_small, b_flags, b_knncol, b_knnt, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
     DevBuf b_scr[8];
+    DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
 };
 
 namespace {
@@ -321,6 +322,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     for (const auto& r : ranges) pairs += (int64_t)r.nq * r.ncand;
     ctx->stats.n_pairs = pairs;
     ctx->stats.n_screen_fallback = 0;
+    ctx->stats.ms_rerank = 0.0;
     const bool screen = prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN;
     if (!screen) {
         ctx->stats.precision_used = IRC_PREC_EXACT;
@@ -383,6 +385,10 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
     CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
     CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
+    CHK(ensure(ctx, ctx->b_cand_n, sizeof(int) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_cand_col, sizeof(int) * (size_t)kprime * nq));
+    CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)kprime * nq));
+    CHK(ensure(ctx, ctx->b_cand_eps, sizeof(double) * (size_t)nq));
     int* fb = ptr<int>(ctx->b_scr[4]);
     int* nfb = fb + nq;
     int* seg_ctr = ptr<int>(ctx->b_scr[5]);
@@ -408,20 +414,26 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.kprime = kprime;
     a.hist = (ctx->flags & 2) ? 1 : 0;
     a.eta = irc::screen_eta(sp, p.D, KP);
-    a.knn_col = knn_col;
-    a.knn_t = knn_t;
-    a.dk2 = dk2;
+    a.n_q = nq;
+    a.cand_n = ptr<int>(ctx->b_cand_n);
+    a.cand_col = ptr<int>(ctx->b_cand_col);
+    a.cand_d = ptr<float>(ctx->b_cand_d);
+    a.cand_eps = ptr<double>(ctx->b_cand_eps);
     a.fallback = fb;
     a.n_fallback = nfb;
+    HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
     HIPCHK(hipEventRecord(ctx->ev[4], st));
     HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, 8), seg_ctr, d_seg_beg));
     HIPCHK(hipEventRecord(ctx->ev[5], st));
+    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, knn_t, dk2));
+    HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2));
     int nf = 0;
     HIPCHK(hipMemcpyAsync(&nf, nfb, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.n_screen_fallback = nf;
+    ctx->stats.ms_rerank = event_ms(ctx->ev[5], ctx->ev[6]);
     return IRC_OK;
 }
 

@@ -13,7 +13,7 @@ struct ScreenItem {
     int ncand;   // candidates of the block (<= IRC_SCREEN_MAXCAND)
 };
 
-#define IRC_SCREEN_MAXCAND 8192   // candidate positions fit 13 bits of the packed key
+#define IRC_SCREEN_MAXCAND 65536  // candidate positions fit 16 bits of the packed key
 #define IRC_SCREEN_MAXRUNS 128    // runs per block: 2 per linked chunk
 
 struct ScreenArgs {
@@ -28,9 +28,13 @@ struct ScreenArgs {
     int kprime;              // over-fetched candidates per query (32, 48 or 64)
     int hist;                // run the histogram pre-pass that sets each query's starting threshold
     double eta;              // |screened - exact| <= eta * (|q|^2 + |c|^2)
-    int* knn_col;
-    double* knn_t;
-    double* dk2;
+    // certified candidate lists, slot-major ([slot][n_q]) so a wavefront's 64 queries write
+    // and the re-rank's lanes read contiguous words
+    int64_t n_q;             // queries of the plan (row stride of the lists)
+    int* cand_n;             // per query: r | (min(k, |C|) << 16), or -1 (not screened / uncertified)
+    int* cand_col;           // [kprime][n_q] candidate columns in ascending screened order
+    float* cand_d;           // [kprime][n_q] their truncated screened distances
+    double* cand_eps;        // per query: the screen's error bound
     int* fallback;           // queries to redo with the exact kernel
     int* n_fallback;
 };
@@ -45,6 +49,10 @@ hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, i
 // per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,
                                const int* seg_beg);
+// FP64 re-rank of the certified lists into the k-lists (knn_col, knn_t, dk2); queries whose
+// ambiguous prefix does not fit the re-rank window join the fallback list.
+hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* knn_t, double* dk2);
+hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 hipError_t launch_mfma_layout_probe(hipStream_t st, float* out);
 
 }  // namespace irc

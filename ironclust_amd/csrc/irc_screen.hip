@@ -289,9 +289,20 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         constexpr int NA = PREC == SCREEN_BF16 ? 2 * KB : 2 * KB;
         float4 An[NA];
         float nnN = 0.0f;
-        auto issue = [&](int pos) {
-            while (run + 1 < nr && pos >= s_ro[run + 1]) ++run;
-            const int col = pos < ncand ? s_rb[run] + (pos - s_ro[run]) : s_rb[0];
+        // tile [pos0, pos0 + 32): `run` (wave-uniform) is the run holding pos0; the few runs that
+        // start inside the tile are walked with uniform branches, each lane keeping its column
+        auto issue = [&](int pos0) {
+            while (run + 1 < nr && pos0 >= __builtin_amdgcn_readfirstlane(s_ro[run + 1])) ++run;
+            const int pos = pos0 + m;
+            int rr = run;
+            int col = __builtin_amdgcn_readfirstlane(s_rb[rr]) + (pos - __builtin_amdgcn_readfirstlane(s_ro[rr]));
+            while (rr + 1 < nr) {
+                const int ro = __builtin_amdgcn_readfirstlane(s_ro[rr + 1]);
+                if (ro >= pos0 + 32) break;
+                ++rr;
+                if (pos >= ro) col = __builtin_amdgcn_readfirstlane(s_rb[rr]) + (pos - ro);
+            }
+            if (pos >= ncand) col = s_rb[0];
             nnN = pos < ncand ? a.norm[col] : 0.0f;
             if constexpr (PREC == SCREEN_BF16) {
                 const float4* rh = reinterpret_cast<const float4*>(
@@ -314,7 +325,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         // for the 32 distances of the lane's query, then after_tile()
         auto stream = [&](auto&& consume, auto&& after_tile) {
             run = 0;
-            issue(m);
+            issue(0);
             for (int base = 0; base < ncand; base += 32) {
                 nmax = fmaxf(nmax, nnN);
                 const float4* Ac = An;
@@ -347,7 +358,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 }
                 // the operand registers are free once the MFMAs have read them: the next tile's
                 // loads are in flight while this tile's distances are consumed
-                if (base + 32 < ncand) issue(base + 32 + m);
+                if (base + 32 < ncand) issue(base + 32);
                 // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
                 auto tile_out = [&](auto full) {
 #pragma unroll
@@ -405,9 +416,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         stream(
             [&](int sbits, int cpos, bool valid) {
                 const uint32_t key = ((uint32_t)max(sbits, 0) & hmask) | (uint32_t)cpos;
-                const bool ok = key < tau && valid;
-                sbuf[(ok ? cnt : 64) * 64 + lane] = key;
-                cnt += ok ? 1 : 0;
+                // slot cnt (< 64: at most 32 buffered when a tile starts) is scratch until counted
+                sbuf[cnt * 64 + lane] = key;
+                cnt += (key < tau && valid) ? 1 : 0;
             },
             [&]() {
                 if (__any(cnt > 32)) compact_half();
@@ -429,71 +440,159 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const double eps = a.eta * ((double)nq_mine + (double)nmax_w) + 1e-30;
             const double qf = ldexp(1.0, -(22 - ib));
             const double dkt = key_dist(sbuf[(kk - 1) * 64 + lane], hmask);
+            // every one of the first kk kept candidates has d^2 <= U
             const double U = dkt * (1.0 + qf) + eps;
             // every candidate outside L has a key at least min(L[K'-1], starting threshold)
             const uint32_t kexcl = sbuf[(kprime - 1) * 64 + lane] < tau_init ? sbuf[(kprime - 1) * 64 + lane] : tau_init;
-            bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
-            int r = kk;
-            while (r < nl && (double)key_dist(sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
-            const float* qrow = p.feat + (int64_t)qc_mine * p.DP;
-            double wt[WIN];
-            int wk[WIN], wc[WIN];
-            int wn = 0, out = 0;
-            double last_t = 0.0;
-            int last_c = -1;
-            for (int j = 0; j < r && ok && out < kk; ++j) {
-                const uint32_t key = sbuf[j * 64 + lane];
-                const int pos = (int)(key & pmask);
-                int lo = 0, hi = nr - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_ro[mid] <= pos) lo = mid; else hi = mid - 1;
-                }
-                const int col = s_rb[lo] + (pos - s_ro[lo]);
-                const double t = exact_d2(qrow, p.feat + (int64_t)col * p.DP, p.DP);
-                const int kc = p.col_key[col];
-                if (wn == WIN) { ok = false; break; }
-                // insert (t, kc, col) into the sorted window
-                bool placed = false;
-#pragma unroll
-                for (int i = WIN - 1; i >= 0; --i) {
-                    if (i <= wn && !placed) {
-                        if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
-                            wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
-                        } else {
-                            wt[i] = t; wk[i] = kc; wc[i] = col;
-                            placed = true;
-                        }
+            const bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
+            if (ok) {
+                // the re-rank needs every kept candidate whose lower bound reaches U
+                int r = kk;
+                while (r < nl && (double)key_dist(sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
+                for (int j = 0; j < r; ++j) {
+                    const uint32_t key = sbuf[j * 64 + lane];
+                    const int pos = (int)(key & pmask);
+                    int lo = 0, hi = nr - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_ro[mid] <= pos) lo = mid; else hi = mid - 1;
                     }
+                    a.cand_col[(int64_t)j * a.n_q + q_mine] = s_rb[lo] + (pos - s_ro[lo]);
+                    a.cand_d[(int64_t)j * a.n_q + q_mine] = key_dist(key, hmask);
                 }
-                ++wn;
-                const double lb = j + 1 < r ? (double)key_dist(sbuf[(j + 1) * 64 + lane], hmask) - eps
-                                            : __builtin_inf();
-                while (wn > 0 && out < kk && wt[0] < lb) {
-                    a.knn_col[(int64_t)q_mine * k + out] = wc[0];
-                    a.knn_t[(int64_t)q_mine * k + out] = wt[0];
-                    last_t = wt[0];
-                    last_c = wc[0];
-                    ++out;
-#pragma unroll
-                    for (int i = 0; i < WIN - 1; ++i) {
-                        if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
-                    }
-                    --wn;
-                }
-            }
-            if (ok && out == kk) {
-                for (int j = kk; j < k; ++j) {
-                    a.knn_col[(int64_t)q_mine * k + j] = last_c;
-                    a.knn_t[(int64_t)q_mine * k + j] = last_t;
-                }
-                a.dk2[q_mine] = last_t;
+                a.cand_n[q_mine] = r | (kk << 16);
+                a.cand_eps[q_mine] = eps;
             } else {
+                a.cand_n[q_mine] = -1;
                 const int f = atomicAdd(a.n_fallback, 1);
                 a.fallback[f] = q_mine;
             }
         }
     }
+}
+
+// FP64 re-rank of the certified lists.  A workgroup takes 64 consecutive queries (one
+// (site, chunk) block, or two: their candidate rows are shared and stay in L2).
+// Phase 1: the four wavefronts split the list slots, each lane computing the canonical FP64
+// distance of one (query, slot) pair into LDS -- independent row gathers, many in flight.
+// Phase 2: lane l of wavefront 0 walks query l's slots in ascending screened order with
+// lower bounds lb_j = d_j - eps; a window of WIN exact distances kept in (d^2, id) order emits
+// its head once the head is below the next slot's lower bound (no later slot can precede it).
+// A window overflow (more than WIN mutually ambiguous candidates) sends the query to the exact
+// kernel.
+constexpr int RR_WAVES = 4;
+
+// canonical d^2 of a query row held in registers and a DP-padded candidate row; NV = DP / 4
+// is known at compile time so every row load is issued before the FMA chain starts
+template <int NV>
+__device__ __forceinline__ double exact_d2_n(const float4 (&a)[NV], const float* __restrict__ x) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    float4 b[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) b[i] = x4[i];
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        double t;
+        t = (double)a[i].x - (double)b[i].x; acc = fma(t, t, acc);
+        t = (double)a[i].y - (double)b[i].y; acc = fma(t, t, acc);
+        t = (double)a[i].z - (double)b[i].z; acc = fma(t, t, acc);
+        t = (double)a[i].w - (double)b[i].w; acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+template <int NV>
+__global__ __launch_bounds__(64 * RR_WAVES) void k_rerank(DevPlan p, int64_t n_q, int kslots,
+                                                         const int* __restrict__ cand_n,
+                                                         const int* __restrict__ cand_col,
+                                                         const float* __restrict__ cand_d,
+                                                         const double* __restrict__ cand_eps,
+                                                         int* __restrict__ knn_col, double* __restrict__ knn_t,
+                                                         double* __restrict__ dk2, int* __restrict__ fallback,
+                                                         int* __restrict__ n_fallback) {
+    extern __shared__ double rr_smem[];
+    double* sT = rr_smem;                                          // [kslots][64]
+    int* sK = reinterpret_cast<int*>(rr_smem + (size_t)kslots * 64); // [kslots][64] spike keys
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
+    // of the query groups, so the candidate rows its groups share stay in its L2
+    const unsigned G = gridDim.x, x = blockIdx.x & 7, per = (G + 7) >> 3, lo8 = G & 7;
+    const unsigned grp = (lo8 == 0 || x < lo8) ? x * per + (blockIdx.x >> 3)
+                                               : lo8 * per + (x - lo8) * (per - 1) + (blockIdx.x >> 3);
+    const int64_t q = (int64_t)grp * 64 + l;
+    const bool qv = q < n_q;
+    const int rn = qv ? cand_n[q] : -1;
+    const int r = rn < 0 ? 0 : (rn & 0xffff);
+    if (qv && r > w) {
+        const float4* q4 = reinterpret_cast<const float4*>(p.feat + (int64_t)p.q_col[q] * p.DP);
+        float4 qr[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) qr[i] = q4[i];
+        for (int j = w; j < r; j += RR_WAVES) {
+            const int col = cand_col[(int64_t)j * n_q + q];
+            sT[j * 64 + l] = exact_d2_n<NV>(qr, p.feat + (int64_t)col * p.DP);
+            sK[j * 64 + l] = p.col_key[col];
+        }
+    }
+    __syncthreads();
+    if (w != 0 || !qv || rn < 0) return;
+    const int kk = rn >> 16;
+    const int k = p.k;
+    const double eps = cand_eps[q];
+    double wt[WIN];
+    int wk[WIN], wc[WIN];
+    int wn = 0, out = 0;
+    double last_t = 0.0;
+    int last_c = -1;
+    bool ok = true;
+    for (int j = 0; j < r && out < kk; ++j) {
+        const double t = sT[j * 64 + l];
+        const int kc = sK[j * 64 + l];
+        const int col = cand_col[(int64_t)j * n_q + q];
+        const double lb = j + 1 < r ? (double)cand_d[(int64_t)(j + 1) * n_q + q] - eps : __builtin_inf();
+        if (wn == WIN) { ok = false; break; }
+        bool placed = false;
+#pragma unroll
+        for (int i = WIN - 1; i >= 0; --i) {
+            if (i <= wn && !placed) {
+                if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
+                    wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
+                } else {
+                    wt[i] = t; wk[i] = kc; wc[i] = col;
+                    placed = true;
+                }
+            }
+        }
+        ++wn;
+        while (wn > 0 && out < kk && wt[0] < lb) {
+            knn_col[q * k + out] = wc[0];
+            knn_t[q * k + out] = wt[0];
+            last_t = wt[0];
+            last_c = wc[0];
+            ++out;
+#pragma unroll
+            for (int i = 0; i < WIN - 1; ++i) {
+                if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
+            }
+            --wn;
+        }
+    }
+    if (ok && out == kk) {
+        for (int j = kk; j < k; ++j) {
+            knn_col[q * k + j] = last_c;
+            knn_t[q * k + j] = last_t;
+        }
+        dk2[q] = last_t;
+    } else {
+        const int f = atomicAdd(n_fallback, 1);
+        fallback[f] = (int)q;
+    }
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_zero_ints(int* p, int n) {
@@ -594,6 +693,31 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArg
         }
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* knn_t, double* dk2) {
+    if (a.n_q <= 0) return hipSuccess;
+    const size_t smem = (size_t)a.kprime * 64 * (sizeof(double) + sizeof(int));
+    const unsigned grid = (unsigned)((a.n_q + 63) / 64);
+#define IRC_RR(NV)                                                                                        \
+    case NV:                                                                                              \
+        k_rerank<NV><<<grid, 64 * RR_WAVES, smem, st>>>(a.p, a.n_q, a.kprime, a.cand_n, a.cand_col, a.cand_d, \
+                                                        a.cand_eps, knn_col, knn_t, dk2, a.fallback,      \
+                                                        a.n_fallback);                                    \
+        break;
+    switch (a.p.DP / 4) {
+        IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)
+        IRC_RR(9) IRC_RR(10) IRC_RR(11) IRC_RR(12) IRC_RR(13) IRC_RR(14) IRC_RR(15) IRC_RR(16)
+        default: return hipErrorInvalidValue;
+    }
+#undef IRC_RR
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v) {
+    if (n <= 0) return hipSuccess;
+    k_fill_int<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_mfma_layout_probe(hipStream_t st, float* out) {

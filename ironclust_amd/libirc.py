@@ -55,7 +55,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("n_queries", ctypes.c_int64), ("n_pairs", ctypes.c_int64),
                 ("n_screen_fallback", ctypes.c_int64), ("n_delta_full", ctypes.c_int64),
                 ("ms_total", ctypes.c_double), ("ms_knn", ctypes.c_double), ("ms_delta", ctypes.c_double),
-                ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32)]
+                ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32),
+                ("ms_rerank", ctypes.c_double)]
 
 
 class IrcError(RuntimeError):

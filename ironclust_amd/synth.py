@@ -91,7 +91,7 @@ def make_dataset(n_spikes, n_sites, D, n_drift, n_clusters, k=30, drift_step=0.0
         rem = counts - nsc.sum(1)
         for s in range(S):
             if rem[s] > 0:
-                nsc[s, rng.integers(0, C, size=rem[s])] += 1
+                np.add.at(nsc[s], rng.integers(0, C, size=rem[s]), 1)
         chunk_tot = nsc.sum(0)
         lim = np.concatenate([[0], np.cumsum(chunk_tot)]).astype(np.int64) + 1
         site_parts = []

@@ -95,3 +95,22 @@ def layout_to_ds(z):
     return dict(n_spikes=N, n_sites=S, D=D, n_drift=len(lim) - 1, k=int(z["k"]), F1=F1,
                 F2=F2 if has2 else None, site1=s1, site2=s2 if has2 else None, lim=lim,
                 link=np.asarray(z["link"]))
+
+
+def subset_sites(ds, sites):
+    """Spike-level sub-recording holding only the spikes whose primary site is in `sites`
+    (primary-only data).  Ids are renumbered in time order and the drift chunks keep their
+    membership, so kNN, rho, delta and nneigh of those spikes are unchanged (ties follow id
+    order, which the renumbering preserves).  Returns (sub, orig_ids) with orig_ids 1-based."""
+    s1 = np.asarray(ds["site1"].cpu() if hasattr(ds["site1"], "cpu") else ds["site1"])
+    keep = np.isin(s1, np.asarray(sites))
+    orig = np.flatnonzero(keep) + 1
+    lim = np.asarray(ds["lim"])
+    chunk = np.searchsorted(lim, orig, side="right") - 1
+    C = len(lim) - 1
+    sub_lim = np.searchsorted(chunk, np.arange(C + 1), side="left") + 1
+    F1 = ds["F1"].cpu().numpy() if hasattr(ds["F1"], "cpu") else np.asarray(ds["F1"])
+    sub = dict(n_spikes=len(orig), n_sites=ds["n_sites"], D=ds["D"], n_drift=C, k=ds["k"],
+               F1=np.ascontiguousarray(F1[orig - 1]), F2=None, site1=s1[orig - 1].astype(np.int32),
+               site2=None, lim=sub_lim.astype(np.int64), link=np.asarray(ds["link"]))
+    return sub, orig

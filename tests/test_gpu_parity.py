@@ -57,6 +57,54 @@ def test_batch_fewer_candidates_than_k(prec):
     _check_batch(ds, prec, "sparse")
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batch_large_blocks_screened(prec, ctx):
+    # blocks of 15k-22k candidates: stream positions need 15 bits of the screen key, so the
+    # truncated keys are coarser; the screen must still certify almost every query
+    ds = synth.make_dataset(45000, 2, 16, 3, 6, k=30, drift_step=0.2, n_links=2, band=1, seed=38)
+    _check_batch(ds, prec, "large-blocks")
+    lay = irc2.site_layout(ds)
+    host = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in lay.items()}
+    ctx.rho_delta_batch(host, 30, opts=libirc.make_options(libirc.PREC_BF16_SCREEN if prec == "bf16"
+                                                           else libirc.PREC_FP32))
+    st = ctx.stats()
+    assert st["precision_used"] == (libirc.PREC_BF16_SCREEN if prec == "bf16" else libirc.PREC_FP32)
+    assert st["n_screen_fallback"] < 0.01 * ds["n_spikes"], st
+
+
+def test_full_size_site_subset_bf16():
+    """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
+    oracle recomputes three whole sites (their results do not depend on other sites)."""
+    import torch
+    from tests.helpers import subset_sites
+    ds = synth.make_workload("drift-10M", device="cuda")
+    lay = irc2.site_layout(ds)
+    N, k = ds["n_spikes"], ds["k"]
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"], n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+    ctx = irc2.get_ctx(0)
+    ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
+    torch.cuda.synchronize()
+    host = dict(ds, F1=ds["F1"].cpu(), site1=ds["site1"].cpu())
+    sub, orig = subset_sites(host, [1, 200, 384])
+    del host
+    r = O.sort_batch(sub)
+    rows = torch.from_numpy(orig - 1).cuda()
+    np.testing.assert_array_equal(out["knn"][rows].cpu().numpy(), orig[r["knn"] - 1])
+    np.testing.assert_array_equal(out["rho"][rows].cpu().numpy(), r["rho"])
+    np.testing.assert_array_equal(out["nneigh"][rows].cpu().numpy(), orig[r["nneigh"] - 1])
+    np.testing.assert_array_equal(out["delta"][rows].cpu().numpy(), r["delta"])
+    # whole-recording properties: self first, denser neighbour, sorted distances
+    ids = torch.arange(1, N + 1, device="cuda")
+    assert bool((out["knn"][:, 0] == ids).all())
+    has = out["nneigh"] != ids
+    assert bool((out["rho"][out["nneigh"][has] - 1] > out["rho"][has]).all())
+
+
 def test_batch_odd_dims_and_k():
     for D, k in ((1, 1), (5, 3), (45, 33), (62, 64)):
         ds = synth.make_dataset(3000, 2, D, 3, 3, k=k, drift_step=0.1, n_links=2, band=1, seed=36 + D)

@@ -250,3 +250,23 @@ def test_golden_fixtures_reproduce():
             ref = brute_batch(ds)
             for key in ("knn", "rho", "delta", "nneigh"):
                 np.testing.assert_array_equal(ref[key], z[key], err_msg=f"{f} {key} brute")
+
+
+def test_site_subset_preserves_results():
+    from tests.helpers import subset_sites
+    ds = synth.make_dataset(6000, 4, 6, 10, 3, k=7, drift_step=0.3, n_links=3, band=2, seed=23)
+    full = O.sort_batch(ds)
+    sub, orig = subset_sites(ds, [2, 4])
+    r = O.sort_batch(sub)
+    np.testing.assert_array_equal(orig[r["knn"] - 1], full["knn"][orig - 1])
+    np.testing.assert_array_equal(r["rho"], full["rho"][orig - 1])
+    np.testing.assert_array_equal(orig[r["nneigh"] - 1], full["nneigh"][orig - 1])
+    np.testing.assert_array_equal(r["delta"], full["delta"][orig - 1])
+
+
+def test_lognormal_generator_total_and_chunks():
+    # per-site lognormal counts and per-chunk Dirichlet shares must add up to the requested total
+    ds = synth.make_dataset(30000, 16, 4, 12, 3, lognormal=True, seed=4)
+    assert ds["site1"].numel() == 30000
+    assert ds["lim"][0] == 1 and ds["lim"][-1] == 30001
+    assert np.all(np.diff(ds["lim"]) >= 0)
