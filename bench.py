@@ -32,6 +32,12 @@ def parse():
     ap.add_argument("--cpu-sites", type=int, default=0, help="sites in the CPU baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="verify size-independent properties of the output")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick k_over / grid / list depth / pre-pass / group size per form by timing them first "
+                         "(outside the timed steps)")
+    ap.add_argument("--per-site", type=int, default=0,
+                    help="also time the per-site call pattern (irc_knn_drift + irc_delta_drift per site, host "
+                         "buffers, the reference's rho_paged_site_/delta_paged_site_ loop) on this many sites")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -73,8 +79,15 @@ def main():
     ctx = libirc.Context(0)
     forms = {}
     results = {}
+    tuned = {}
     for form in a.forms.split(","):
         opts = libirc.make_options(precision=PREC[form])
+        if a.autotune and form != "exact":
+            from ironclust_amd import autotune
+            choice, rep = autotune.tune(ctx, dev, out, precision=PREC[form])
+            tuned[form] = {"options": choice, **rep}
+            opts = libirc.make_options(precision=PREC[form], **choice)
+            print(f"# autotune {form}: {choice}", file=sys.stderr, flush=True)
         for _ in range(a.warmup):
             ctx.rho_delta_batch_device(dev, out, opts)
         torch.cuda.synchronize()
@@ -96,10 +109,14 @@ def main():
             "ms_device_total": st_acc["ms_total"] / a.steps,
             "knn_kernel_ms": ms_kernel, "rerank_ms": st_acc["ms_rerank"] / a.steps,
             "knn_tflops": flops / (ms_kernel * 1e-3) / 1e12,
-            "screen_fallback": int(s["n_screen_fallback"]), "delta_full_scan": int(s["n_delta_full"]),
+            "screen_fallback": int(s["n_screen_fallback"]), "rerank_overflow": int(s["n_rerank_overflow"]),
+            "delta_full_scan": int(s["n_delta_full"]),
             "precision_used": int(s["precision_used"]),
         }
-        results[form] = {kk: v.clone() for kk, v in out.items()}
+        if len(a.forms.split(",")) > 1:
+            results[form] = {kk: v.clone() for kk, v in out.items()}
+        else:
+            results[form] = out
         print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
     head = a.forms.split(",")[0]
     # all forms must agree bit for bit
@@ -131,9 +148,56 @@ def main():
                      "flops_per_launch": flops},
         "forms": forms,
     }
+    if tuned:
+        line["autotune"] = tuned
+    if a.per_site > 0:
+        line["per_site_launches"] = per_site(ds, lay, a, ctx, PREC[head], results[head])
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ds, a)
     print(json.dumps(line), flush=True)
+
+
+def per_site(ds, lay, a, ctx, prec, ref):
+    """The per-site call pattern on the first `a.per_site` sites: for each site one
+    irc_knn_drift over its primary columns, then (after every site's rho is known) one
+    irc_delta_drift, with host buffers -- the loop of rho_paged_site_ / delta_paged_site_
+    (matlab/irc2.m:3635-3726) on the library's per-site entry points.  Results are checked
+    against the batched call."""
+    import numpy as np
+    import torch
+    from ironclust_amd import libirc
+    S = min(a.per_site, ds["n_sites"])
+    F = lay["F"].cpu().numpy()
+    col_spike = lay["col_spike"].cpu().numpy()
+    site_ptr, site_np = lay["site_ptr"], lay["site_n_primary"]
+    lim = np.asarray(ds["lim"], dtype=np.int64)
+    link = np.asarray(ds["link"], dtype=np.uint8)
+    k = ds["k"]
+    opts = libirc.make_options(precision=prec)
+    rho_ref = ref["rho"].cpu().numpy()
+    sites = []
+    for s in range(S):
+        b, e = int(site_ptr[s]), int(site_ptr[s + 1])
+        ids = col_spike[b:e].astype(np.int64)
+        drift = (np.searchsorted(lim, ids, side="right")).astype(np.int32)
+        sites.append((np.ascontiguousarray(F[b:e]), drift, ids, int(site_np[s])))
+    n_spk = sum(x[3] for x in sites)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    rhos = []
+    for Fs, drift, ids, npri in sites:
+        dk, _ = ctx.knn_drift(Fs, drift, link, 0, npri, k, id_of_col=ids, opts=opts)
+        rhos.append((1.0 / dk.astype(np.float64)).astype(np.float32))
+    for (Fs, drift, ids, npri), rho in zip(sites, rhos):
+        rho_cols = rho_ref[ids - 1] if len(ids) > npri else rho
+        ctx.delta_drift(Fs, drift, link, 0, npri, rho_cols, id_of_col=ids, opts=opts)
+    dt = time.perf_counter() - t
+    # the per-site entry returns d_k in single precision, so 1/d_k may differ from the batched
+    # rho (computed from d_k^2 in FP64) in the last bit
+    rel = max(float(np.max(np.abs(r / rho_ref[ids[:npri] - 1] - 1.0))) for (Fs, drift, ids, npri), r in
+              zip(sites, rhos))
+    return {"value": n_spk / dt, "unit": "spikes/s", "sites": S, "spikes": int(n_spk), "seconds": dt,
+            "rho_max_rel_diff_vs_batched": rel}
 
 
 def cpu_baseline(ds, a):

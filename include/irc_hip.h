@@ -60,6 +60,14 @@ typedef enum {
     IRC_MEM_DEVICE = 1   /* pointers are device memory of the context's device */
 } irc_mem;
 
+/* irc_options.flags */
+#define IRC_FLAG_HIST 2   /* run the screen's histogram pre-pass (a first pass that sets each query's
+                             starting threshold; off by default) */
+#define IRC_FLAG_GRAPH 4  /* irc_rho_delta_batch: capture the device phase (kNN, rho, delta) in a HIP
+                             graph and replay it on later calls with the same descriptor: same
+                             pointers, sizes, options and host descriptor contents, and unchanged
+                             col_spike contents (only F may change between calls) */
+
 typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
     int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 40, 48 or 64
@@ -68,8 +76,9 @@ typedef struct {
                            leftovers of a block) use the exact kernel (0 = never) */
     int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
     int32_t mem;        /* irc_mem for irc_rho_delta_batch */
-    int32_t flags;      /* bit 1: run the screen's histogram pre-pass (a first pass that sets each
-                           query's starting threshold; off by default) */
+    int32_t flags;      /* IRC_FLAG_* bits */
+    int32_t waves_per_cu; /* persistent grid of the screen, wavefronts per CU (0 = 8) */
+    int32_t large_ncand;  /* blocks with more candidates keep 64 list slots (0 = 8192) */
 } irc_options;
 
 /* Fill *opt with defaults. */
@@ -181,6 +190,8 @@ typedef struct {
     double ms_screen;         /* the screen kernel alone (0 on the exact path) */
     int32_t precision_used;
     double ms_rerank;         /* the FP64 re-rank of the certified lists (0 on the exact path) */
+    int64_t n_rerank_overflow;/* of n_screen_fallback: queries whose ambiguous run overflowed the re-rank window */
+    double ms_graph;          /* device time of the replayed graph (IRC_FLAG_GRAPH replays; else 0) */
 } irc_stats;
 
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);

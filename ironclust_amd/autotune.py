@@ -1,0 +1,92 @@
+"""Autotuner for irc_rho_delta_batch: picks the screen's over-fetch depth (k_over), its
+persistent grid (waves_per_cu), the threshold above which a block keeps 64 list slots
+(large_ncand), the histogram pre-pass and the smallest query group screened per wavefront
+(tile: the queue granularity) by timing the real call on the caller's data, and caches the
+choice in a JSON file keyed by the shape of the workload.
+
+Every candidate configuration must reproduce the default configuration's outputs bit for bit
+(they only change how the exact result is reached); a configuration that does not is dropped.
+"""
+import json
+import os
+
+from . import libirc
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "irc_hip_autotune.json")
+
+GRID = {
+    libirc.PREC_BF16_SCREEN: [
+        dict(k_over=48), dict(k_over=40), dict(k_over=64),
+        dict(k_over=48, waves_per_cu=16), dict(k_over=48, flags=2),
+        dict(k_over=48, large_ncand=4096), dict(k_over=48, tile=8),
+    ],
+    libirc.PREC_FP32: [
+        dict(k_over=40), dict(k_over=32), dict(k_over=48),
+        dict(k_over=40, waves_per_cu=16), dict(k_over=40, tile=8),
+    ],
+}
+
+
+def signature(dev, precision):
+    """Cache key: precision, D, k and the sizes that set the screen's work balance."""
+    F = dev["F"]
+    n = int(dev["n_spikes"])
+    S = len(dev["site_n_primary"])
+    C = len(dev["lim_drift"]) - 1
+    return f"p{precision}-D{int(F.shape[1])}-k{int(dev['k'])}-S{S}-C{C}-N{n >> 16}"
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(path, data):
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def tune(ctx, dev, out, precision=libirc.PREC_BF16_SCREEN, reps=2, cache=DEFAULT_CACHE, grid=None, force=False):
+    """Return (options dict, report).  dev/out: device tensors as for Context.rho_delta_batch_device.
+    Timings are the library's own device time (HIP events around the call's kernels)."""
+    import torch
+    key = signature(dev, precision)
+    db = _load(cache) if cache else {}
+    if not force and key in db:
+        return dict(db[key]["options"]), {"cached": True, "key": key}
+    cands = grid or GRID.get(precision, [dict()])
+    ref = None
+    report = []
+    best = None
+    for c in cands:
+        opts = libirc.make_options(precision=precision, **c)
+        ctx.rho_delta_batch_device(dev, out, opts)          # warm-up (buffers, code objects)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = {kk: v.clone() for kk, v in out.items()}
+            same = True
+        else:
+            same = all(torch.equal(out[kk], ref[kk]) for kk in ref)
+        ms = []
+        for _ in range(reps):
+            ctx.rho_delta_batch_device(dev, out, opts)
+            ms.append(ctx.stats()["ms_total"])
+        t = min(ms)
+        report.append({"options": c, "ms": t, "identical": same})
+        if same and (best is None or t < best[1]):
+            best = (c, t)
+    if cache:
+        db[key] = {"options": best[0], "ms": best[1], "trials": report}
+        try:
+            _save(cache, db)
+        except OSError:
+            pass                      # a read-only home: the choice still applies to this run
+    return dict(best[0]), {"cached": False, "key": key, "trials": report}

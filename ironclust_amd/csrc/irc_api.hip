@@ -16,6 +16,7 @@
 
 #include "irc_common.h"
 #include "irc_kernels.h"
+#include "irc_plan.h"
 #include "irc_screen.h"
 #include "../../include/irc_hip.h"
 
@@ -36,12 +37,21 @@ struct irc_ctx {
     std::string err;
     irc_stats stats{};
     int flags = 0;       // irc_options.flags of the current call
+    int waves_per_cu = 8;  // persistent grid of the screen
+    int large_ncand = 8192;
     // device buffers, grown on demand and kept across calls
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
     DevBuf b_small, b_flags, b_knncol, b_knnt, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
     DevBuf b_scr[8];
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
+    DevBuf b_plan;                                         // device-side work planning
+    DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
+                          // re-rank, [2] pairs, [3] queries whose denser neighbour was outside the kNN list
+    bool dstat_screen = false;  // [0], [1] hold this call's screen counters
+    bool capturing = false;     // the stream is being captured into a graph: no events, no host copies
+    hipGraphExec_t graph_exec = nullptr;  // the batch call's device phase, for replays
+    uint64_t graph_key = 0;
 };
 
 namespace {
@@ -98,6 +108,9 @@ int pad_dim(int D) { return (D + 3) / 4 * 4; }
 irc_status check_opt(irc_ctx* ctx, const irc_options* opt) {
     if (opt && opt->force_fail) return fail(ctx, IRC_E_FORCED, "failure forced by irc_options.force_fail");
     ctx->flags = opt ? opt->flags : 0;
+    ctx->waves_per_cu = opt && opt->waves_per_cu > 0 ? opt->waves_per_cu : 8;
+    ctx->large_ncand = opt && opt->large_ncand > 0 ? opt->large_ncand : 8192;
+    if (ctx->waves_per_cu > 32) return fail(ctx, IRC_E_ARG, "waves_per_cu=%d above 32", ctx->waves_per_cu);
     return IRC_OK;
 }
 
@@ -173,6 +186,8 @@ irc_status upload_plan(irc_ctx* ctx, const HostPlan& h, DevPlan& p) {
     p.q_col = ptr<int>(ctx->b_qcol);
     p.q_blk = ptr<int>(ctx->b_qblk);
     p.q_out = ptr<int>(ctx->b_qout);
+    p.nq = (int)h.qcol.size();
+    p.n_blocks = (int)(h.runb.size() / 2);
     return IRC_OK;
 }
 
@@ -312,68 +327,116 @@ void block_ranges_cands(int C, const std::vector<int>& runl, const std::vector<i
     }
 }
 
+// Device-side planning inputs of the batch path (the block runs are already on the device).
+struct DevWork {
+    int S, C;
+    const int64_t* site_ptr;   // device [S+1]
+    const int64_t* prim;       // device [S+1] prefix of primary counts
+    int64_t max_site_cols;     // largest column count of a site: bounds every block's candidates
+};
+
 // kNN pass: MFMA screen + certification + FP64 re-rank for every block that fits the
-// screen, the exact FP64 kernel for the rest and for uncertified queries.
+// screen, the exact FP64 kernel for the rest and for uncertified queries.  The work list comes
+// from `ranges` (host planning, per-site entry points) or is built on the device (`dw`, batch).
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
-                   int FS, int64_t n_cols, const std::vector<QRange>& ranges, int* knn_col, double* knn_t,
-                   double* dk2) {
+                   int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
+                   double* knn_t, double* dk2) {
     hipStream_t st = ctx->stream;
     int64_t pairs = 0;
-    for (const auto& r : ranges) pairs += (int64_t)r.nq * r.ncand;
+    if (ranges)
+        for (const auto& r : *ranges) pairs += (int64_t)r.nq * r.ncand;
     ctx->stats.n_pairs = pairs;
     ctx->stats.n_screen_fallback = 0;
     ctx->stats.ms_rerank = 0.0;
-    const bool screen = prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN;
+    ctx->stats.n_rerank_overflow = 0;
+    ctx->dstat_screen = false;
+    CHK(ensure(ctx, ctx->b_dstat, sizeof(int64_t) * 8));
+    HIPCHK(irc::launch_zero_i64(st, ptr<int64_t>(ctx->b_dstat), 8));
+    // the screen's operand rows hold D + 2 values in at most 4 MFMA K-blocks of 16 (D <= 62);
+    // wider features take the exact path
+    const bool screen = (prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN) && irc::screen_kp(p.D) <= 64;
     if (!screen) {
         ctx->stats.precision_used = IRC_PREC_EXACT;
-        HIPCHK(hipEventRecord(ctx->ev[4], st));
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
         HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2, nullptr));
-        HIPCHK(hipEventRecord(ctx->ev[5], st));
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
         ctx->stats.n_screen_fallback = nq;
         return IRC_OK;
     }
     ctx->stats.precision_used = prec;
+    ctx->dstat_screen = true;
     const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : irc::SCREEN_BF16;
     // over-fetch: the BF16 form's looser bound needs a deeper list to certify the k-th
     int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_BF16 ? 48 : 40);
     if (kprime < p.k) kprime = p.k;
     kprime = kprime <= 32 ? 32 : (kprime <= 40 ? 40 : (kprime <= 48 ? 48 : 64));
-    // items (<= 64 queries each) in site-major order; queries outside the screen's limits go exact
-    std::vector<irc::ScreenItem> items;
-    std::vector<int> exact_q;
-    std::vector<double> cost;
-    for (const auto& r : ranges) {
-        if (r.nq <= 0) continue;
-        if (r.ncand > IRC_SCREEN_MAXCAND || r.nruns > IRC_SCREEN_MAXRUNS) {
-            for (int i = 0; i < r.nq; ++i) exact_q.push_back(r.q0 + i);
-            continue;
-        }
-        for (int i = 0; i < r.nq; i += 64) {
-            const int n = r.nq - i < 64 ? r.nq - i : 64;
-            if (n < min_item) {
-                // a wavefront streams the whole candidate set whatever its query count: a few
-                // leftover queries cost less on the exact kernel
-                for (int j = 0; j < n; ++j) exact_q.push_back(r.q0 + i + j);
+    const int large_ncand = ctx->large_ncand;
+    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
+    CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
+    int* fb = ptr<int>(ctx->b_scr[4]);
+    int* nfb = fb + nq;
+    int* seg_ctr = ptr<int>(ctx->b_scr[5]);
+    int* d_seg_beg = seg_ctr + 8;
+    int n_items_cap = 0;
+    bool any_large = false;
+    irc::PlanBuf pb{};
+    if (ranges) {
+        // items (<= 64 queries each) in site-major order; queries outside the screen's limits go exact
+        std::vector<irc::ScreenItem> items;
+        std::vector<int> exact_q;
+        std::vector<double> cost;
+        for (const auto& r : *ranges) {
+            if (r.nq <= 0) continue;
+            if (r.ncand > IRC_SCREEN_MAXCAND || r.nruns > IRC_SCREEN_MAXRUNS) {
+                for (int i = 0; i < r.nq; ++i) exact_q.push_back(r.q0 + i);
                 continue;
             }
-            items.push_back({r.blk, r.q0 + i, n, r.ncand});
-            cost.push_back((double)r.ncand + 64.0);
+            for (int i = 0; i < r.nq; i += 64) {
+                const int n = r.nq - i < 64 ? r.nq - i : 64;
+                if (n < min_item) {
+                    // a wavefront streams the whole candidate set whatever its query count: a few
+                    // leftover queries cost less on the exact kernel
+                    for (int j = 0; j < n; ++j) exact_q.push_back(r.q0 + i + j);
+                    continue;
+                }
+                items.push_back({r.blk, r.q0 + i, n, r.ncand});
+                cost.push_back((double)r.ncand + 64.0);
+                any_large = any_large || r.ncand > large_ncand;
+            }
         }
-    }
-    // eight contiguous segments of about equal cost, one per XCD
-    int seg_beg[9];
-    {
-        double tot = 0.0;
-        for (double c : cost) tot += c;
-        double acc = 0.0;
-        int sgi = 1;
-        seg_beg[0] = 0;
-        for (size_t i = 0; i < items.size() && sgi < 8; ++i) {
-            acc += cost[i];
-            while (sgi < 8 && acc >= tot * sgi / 8.0) seg_beg[sgi++] = (int)i + 1;
+        // eight contiguous segments of about equal cost, one per XCD
+        int seg_beg[9];
+        {
+            double tot = 0.0;
+            for (double c : cost) tot += c;
+            double acc = 0.0;
+            int sgi = 1;
+            seg_beg[0] = 0;
+            for (size_t i = 0; i < items.size() && sgi < 8; ++i) {
+                acc += cost[i];
+                while (sgi < 8 && acc >= tot * sgi / 8.0) seg_beg[sgi++] = (int)i + 1;
+            }
+            while (sgi < 8) seg_beg[sgi++] = (int)items.size();
+            seg_beg[8] = (int)items.size();
         }
-        while (sgi < 8) seg_beg[sgi++] = (int)items.size();
-        seg_beg[8] = (int)items.size();
+        n_items_cap = (int)items.size();
+        CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
+        const int n_exact = (int)exact_q.size();
+        if (!items.empty())
+            HIPCHK(hipMemcpyAsync(ctx->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
+                                  hipMemcpyHostToDevice, st));
+        if (n_exact) HIPCHK(hipMemcpyAsync(fb, exact_q.data(), sizeof(int) * n_exact, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(nfb, &n_exact, sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_seg_beg, seg_beg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
+    } else {
+        const int64_t nb = (int64_t)dw->S * dw->C;
+        n_items_cap = nq / 64 + (int)nb + 1;
+        CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (size_t)n_items_cap));
+        CHK(ensure(ctx, ctx->b_plan, irc::plan_buf_bytes(nb)));
+        pb = irc::plan_buf_carve(ctx->b_plan.p, nb);
+        HIPCHK(irc::launch_plan(st, dw->S, dw->C, p, dw->site_ptr, dw->prim, min_item, IRC_SCREEN_MAXCAND,
+                                IRC_SCREEN_MAXRUNS, pb, ptr<irc::ScreenItem>(ctx->b_scr[3]), fb, d_seg_beg, nfb));
+        any_large = dw->max_site_cols > large_ncand;
     }
     const int KP = irc::screen_kp(p.D);
     const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
@@ -381,37 +444,30 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
     CHK(ensure(ctx, ctx->b_scr[6], eb * (size_t)n_cols * KP));
     CHK(ensure(ctx, ctx->b_scr[7], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
-    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols));
-    CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
-    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
-    CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
+    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
+    // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
+    const int slots = (kprime > 32 && any_large) ? 64 : kprime;
     CHK(ensure(ctx, ctx->b_cand_n, sizeof(int) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_cand_col, sizeof(int) * (size_t)kprime * nq));
-    CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)kprime * nq));
+    CHK(ensure(ctx, ctx->b_cand_col, sizeof(int) * (size_t)slots * nq));
+    CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->b_cand_eps, sizeof(double) * (size_t)nq));
-    int* fb = ptr<int>(ctx->b_scr[4]);
-    int* nfb = fb + nq;
-    int* seg_ctr = ptr<int>(ctx->b_scr[5]);
-    int* d_seg_beg = seg_ctr + 8;
-    const int n_exact = (int)exact_q.size();
-    if (!items.empty())
-        HIPCHK(hipMemcpyAsync(ctx->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
-                              hipMemcpyHostToDevice, st));
-    if (n_exact) HIPCHK(hipMemcpyAsync(fb, exact_q.data(), sizeof(int) * n_exact, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(nfb, &n_exact, sizeof(int), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d_seg_beg, seg_beg, sizeof(int) * 9, hipMemcpyHostToDevice, st));
     HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
                                    ctx->b_scr[6].p, ctx->b_scr[7].p, ptr<float>(ctx->b_scr[2])));
+    float* run_nmax = ptr<float>(ctx->b_scr[2]) + n_cols;
+    HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->b_scr[2]),
+                                run_nmax));
     irc::ScreenArgs a;
     a.p = p;
+    a.run_nmax = run_nmax;
     a.hi = ctx->b_scr[0].p;
     a.lo = ctx->b_scr[1].p;
     a.qhi = ctx->b_scr[6].p;
     a.qlo = ctx->b_scr[7].p;
     a.norm = ptr<float>(ctx->b_scr[2]);
     a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
-    a.n_items = (int)items.size();
+    a.n_items = n_items_cap;
     a.kprime = kprime;
+    a.large_ncand = large_ncand;
     a.hist = (ctx->flags & 2) ? 1 : 0;
     a.eta = irc::screen_eta(sp, p.D, KP);
     a.n_q = nq;
@@ -422,18 +478,35 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.fallback = fb;
     a.n_fallback = nfb;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
-    HIPCHK(hipEventRecord(ctx->ev[4], st));
-    HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, 8), seg_ctr, d_seg_beg));
-    HIPCHK(hipEventRecord(ctx->ev[5], st));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
+    HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
+    int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
+    HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
     HIPCHK(irc::launch_screen_rerank(st, a, knn_col, knn_t, dk2));
-    HIPCHK(hipEventRecord(ctx->ev[6], st));
+    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2));
-    int nf = 0;
-    HIPCHK(hipMemcpyAsync(&nf, nfb, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    ctx->stats.n_screen_fallback = nf;
-    ctx->stats.ms_rerank = event_ms(ctx->ev[5], ctx->ev[6]);
+    HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 1));
+    if (!ranges) HIPCHK(irc::launch_stat_copy(st, nullptr, pb.n_pairs, dstat + 2));
+    return IRC_OK;
+}
+
+// After the call's stream has been synchronized: device counters and kernel times into stats.
+irc_status collect_stats(irc_ctx* ctx, bool events) {
+    int64_t d[8] = {0};
+    if (ctx->b_dstat.p) HIPCHK(hipMemcpy(d, ctx->b_dstat.p, sizeof(d), hipMemcpyDeviceToHost));
+    if (ctx->dstat_screen) {
+        ctx->stats.n_screen_fallback = d[1];
+        ctx->stats.n_rerank_overflow = d[1] - d[0];
+        if (d[2] > 0) ctx->stats.n_pairs = d[2];
+        if (events) {
+            ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
+            ctx->stats.ms_rerank = event_ms(ctx->ev[5], ctx->ev[6]);
+        }
+    } else if (events) {
+        ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
+    }
     return IRC_OK;
 }
 
@@ -489,11 +562,13 @@ void irc_ctx_destroy(irc_ctx* ctx) {
                      &ctx->b_runl, &ctx->b_lptr, &ctx->b_lidx, &ctx->b_qcol, &ctx->b_qblk, &ctx->b_qout,
                      &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_knnt, &ctx->b_dk2, &ctx->b_rhoslot,
                      &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
-                     &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand};
+                     &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
+                     &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan};
     for (DevBuf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->b_scr)
         if (b.p) (void)hipFree(b.p);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -551,8 +626,8 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     }
     block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols, ranges,
-                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols,
+                &ranges, nullptr, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
@@ -560,7 +635,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     HIPCHK(hipMemcpyAsync(knn_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.ms_knn = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
-    ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
+    CHK(collect_stats(ctx, true));
     return IRC_OK;
 }
 
@@ -715,14 +790,15 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     p.q_col = ptr<int>(ctx->b_qcol);
     p.q_blk = ptr<int>(ctx->b_qblk);
     p.q_out = ptr<int>(ctx->b_qout);
+    p.nq = (int)N;
+    p.n_blocks = S * C;
     int* d_flags = ptr<int>(ctx->b_flags);
     int* d_nunres = ptr<int>(ctx->b_unres) + N;
     const int nq = (int)N;
 
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = N;
-    HIPCHK(hipMemsetAsync(d_flags, 0, sizeof(int) * 2, st));
-    HIPCHK(hipMemsetAsync(d_nunres, 0, sizeof(int), st));
+    HIPCHK(irc::launch_fill_int(st, d_flags, 2, 0));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     HIPCHK(irc::launch_pack_fp32(st, n_cols, D, DP, dF, ptr<float>(ctx->b_feat)));
     HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
@@ -735,34 +811,86 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
     if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
     if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
-    // per-block query ranges and candidate counts (the block runs computed on the device)
-    std::vector<int> runb((size_t)S * C * 2), runl((size_t)S * C * 2);
-    HIPCHK(hipMemcpyAsync(runb.data(), ctx->b_runb.p, sizeof(int) * runb.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(runl.data(), ctx->b_runl.p, sizeof(int) * runl.size(), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<QRange> ranges;
-    ranges.reserve((size_t)S * C);
-    for (int s = 0; s < S; ++s)
-        for (int c = 0; c < C; ++c) {
-            const int bb = s * C + c;
-            if (runl[2 * bb] == 0) continue;
-            ranges.push_back({bb, (int)(prim[s] + (runb[2 * bb] - b->site_ptr[s])), runl[2 * bb], 0, 0});
+    // the work list is built on the device from the block runs (no host round trip)
+    DevWork dw;
+    dw.S = S;
+    dw.C = C;
+    dw.site_ptr = d_site_ptr;
+    dw.prim = d_prim;
+    dw.max_site_cols = 0;
+    for (int s = 0; s < S; ++s) dw.max_site_cols = std::max<int64_t>(dw.max_site_cols, b->site_ptr[s + 1] - b->site_ptr[s]);
+    // the device phase: kNN (plan, pack, screen, re-rank, exact), rho, delta.  With
+    // IRC_FLAG_GRAPH it is captured once into a HIP graph and replayed while the call's
+    // descriptor (pointers, sizes, options, host descriptor contents) is unchanged.
+    auto device_phase = [&]() -> irc_status {
+        HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
+        CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
+                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+        HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
+                                      o_knn, nullptr));
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
+        HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
+        HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
+                                          ptr<float>(ctx->b_rhocol), o_delta, o_nn, ptr<int>(ctx->b_unres),
+                                          d_nunres));
+        HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
+                                       ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
+        HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->b_dstat) + 3));
+        return IRC_OK;
+    };
+    const bool graph = opt && (opt->flags & IRC_FLAG_GRAPH);
+    bool replayed = false;
+    if (graph) {
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](const void* v, size_t n) {
+            const unsigned char* c = reinterpret_cast<const unsigned char*>(v);
+            for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+        };
+        const void* ptrs[] = {dF, dspk, o_rho, o_delta, o_nn, o_knn, o_dk};
+        const int64_t sizes[] = {N, S, C, D, k, n_cols, dev ? 1 : 0};
+        mix(ptrs, sizeof(ptrs));
+        mix(sizes, sizeof(sizes));
+        mix(opt, sizeof(*opt));
+        mix(b->site_ptr, sizeof(int64_t) * (S + 1));
+        mix(b->site_n_primary, sizeof(int64_t) * S);
+        mix(b->lim_drift, sizeof(int64_t) * (C + 1));
+        mix(b->link, (size_t)C * C);
+        if (ctx->graph_exec && ctx->graph_key == h) {
+            HIPCHK(hipEventRecord(ctx->ev[1], st));
+            HIPCHK(hipGraphLaunch(ctx->graph_exec, st));
+            replayed = true;
+        } else {
+            CHK(device_phase());
+            // record the same work (all buffers now exist) for the next call with this descriptor
+            if (ctx->graph_exec) {
+                (void)hipGraphExecDestroy(ctx->graph_exec);
+                ctx->graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            ctx->capturing = true;
+            const irc_status cs = device_phase();
+            ctx->capturing = false;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (cs != IRC_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            if (ce != hipSuccess) return fail(ctx, IRC_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            const hipError_t ie = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                ctx->graph_exec = nullptr;
+                return fail(ctx, IRC_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+            }
+            ctx->graph_key = h;
         }
-    block_ranges_cands(C, runl, lptr, lidx, ranges);
-    HIPCHK(hipEventRecord(ctx->ev[1], st));
-    CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, ranges,
-                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
-    HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk, o_knn,
-                                  nullptr));
-    HIPCHK(hipEventRecord(ctx->ev[2], st));
-    HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
-    HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
-                                      ptr<float>(ctx->b_rhocol), o_delta, o_nn, ptr<int>(ctx->b_unres), d_nunres));
-    HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
-                                   ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
+    } else {
+        CHK(device_phase());
+    }
     HIPCHK(hipEventRecord(ctx->ev[3], st));
-    int nunres = 0;
-    HIPCHK(hipMemcpyAsync(&nunres, d_nunres, sizeof(int), hipMemcpyDeviceToHost, st));
+
     if (!dev) {
         HIPCHK(hipMemcpyAsync(out->rho, o_rho, sizeof(float) * N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(out->delta, o_delta, sizeof(float) * N, hipMemcpyDeviceToHost, st));
@@ -771,10 +899,17 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         if (out->dk) HIPCHK(hipMemcpyAsync(out->dk, o_dk, sizeof(float) * N, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
-    ctx->stats.n_delta_full = nunres;
-    ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
-    ctx->stats.ms_knn = event_ms(ctx->ev[1], ctx->ev[2]);
-    ctx->stats.ms_delta = event_ms(ctx->ev[2], ctx->ev[3]);
+    CHK(collect_stats(ctx, !replayed));
+    {
+        int64_t d3 = 0;
+        HIPCHK(hipMemcpy(&d3, ptr<int64_t>(ctx->b_dstat) + 3, sizeof(d3), hipMemcpyDeviceToHost));
+        ctx->stats.n_delta_full = d3;
+    }
+    if (!replayed) {
+        ctx->stats.ms_knn = event_ms(ctx->ev[1], ctx->ev[2]);
+        ctx->stats.ms_delta = event_ms(ctx->ev[2], ctx->ev[3]);
+    }
+    ctx->stats.ms_graph = replayed ? event_ms(ctx->ev[1], ctx->ev[3]) : 0.0;
     ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[3]);
     return IRC_OK;
 }
@@ -802,8 +937,9 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
-    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, {}, ptr<int>(ctx->b_knncol),
-                ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+    const std::vector<QRange> no_ranges;
+    CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, &no_ranges, nullptr,
+                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));

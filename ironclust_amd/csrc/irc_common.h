@@ -22,10 +22,12 @@ struct DevPlan {
     int k;
     const int* col_key;     // tie-break key per column (spike id or column index)
     const int* col_ret;     // value reported for a column (1-based spike id or column)
+    int n_blocks;           // (site, chunk) blocks
     const int* run_beg;     // [n_blocks*2]
     const int* run_len;     // [n_blocks*2]
     const int* link_ptr;    // [C+1]
     const int* link_idx;    // chunks r with link(r, c), ascending
+    int nq;                 // queries of the plan; the kNN lists are slot-major, [k][nq]
     const int* q_col;       // [nq] query column
     const int* q_blk;       // [nq] block of the query (site*C + chunk)
     const int* q_out;       // [nq] output slot

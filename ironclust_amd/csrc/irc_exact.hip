@@ -16,20 +16,6 @@
 
 namespace {
 
-__device__ __forceinline__ double dist2_lds(const double* __restrict__ sq, const float* __restrict__ x, int DP) {
-    double acc = 0.0;
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    for (int d4 = 0; d4 < DP / 4; ++d4) {
-        float4 v = x4[d4];
-        double t;
-        t = sq[4 * d4 + 0] - (double)v.x; acc = fma(t, t, acc);
-        t = sq[4 * d4 + 1] - (double)v.y; acc = fma(t, t, acc);
-        t = sq[4 * d4 + 2] - (double)v.z; acc = fma(t, t, acc);
-        t = sq[4 * d4 + 3] - (double)v.w; acc = fma(t, t, acc);
-    }
-    return acc;
-}
-
 // wave-wide argmin of (t, key) carrying col; ties of identical (t,key) go to the lower lane
 __device__ __forceinline__ void wave_argmin(double& t, int& key, int& col, int& lane) {
 #pragma unroll
@@ -51,18 +37,22 @@ __device__ __forceinline__ bool less_tc(const DevPlan& p, double t, int key, dou
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Exact kNN: one wavefront per query.  Lanes stride over the candidate columns of
-// the linked chunks, each lane keeps its own sorted list of up to k entries in LDS,
-// then k rounds of a wave-wide argmin merge the 64 lists.
-// Writes knn_col / knn_t (query-major, k per query) and dk2 per query.
+// Exact kNN: one wavefront per query.  The candidate runs of the linked chunks are consecutive
+// rows of feat, streamed 64 rows at a time straight into LDS (global_load_lds: each load
+// instruction moves 1 KB of consecutive bytes); lane j takes row j of the batch against the
+// query row held in registers, and keeps its own sorted list of up to k entries in LDS; k
+// rounds of a wave-wide argmin then merge the 64 lists.  The next batch is in flight while a
+// batch is scored.
+// Writes knn_col / knn_t (slot-major: entry j of query q at j * p.nq + q) and dk2 per query.
+template <int NV>
 __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restrict__ qlist, int nq_host,
                                                   const int* __restrict__ nq_dev,
                                                   int* __restrict__ knn_col, double* __restrict__ knn_t,
                                                   double* __restrict__ dk2, int* __restrict__ n_cand) {
     extern __shared__ double smem[];
-    double* sq = smem;                                    // [IRC_DMAX]
-    double* lt = smem + IRC_DMAX;                         // [k][64] distances
-    int* lc = reinterpret_cast<int*>(lt + p.k * 64);      // [k][64] columns (key = col_key[col])
+    float4* stage = reinterpret_cast<float4*>(smem);                  // [64][NV] rows of the batch
+    double* lt = smem + 64 * NV * 2;                                   // [k][64] distances
+    int* lc = reinterpret_cast<int*>(lt + p.k * 64);                   // [k][64] columns (key = col_key[col])
     const int lane = threadIdx.x;
     const int k = p.k;
     const int nq = nq_dev ? *nq_dev : nq_host;
@@ -71,25 +61,86 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
         const int qc = p.q_col[q];
         const int blk = p.q_blk[q];
         const int s = blk / p.C, c = blk - s * p.C;
-        __syncthreads();
-        if (lane < p.DP) sq[lane] = (double)p.feat[(int64_t)qc * p.DP + lane];
-        __syncthreads();
-        int n = 0;
-        int total = 0;
-        for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
-            const int b2 = s * p.C + p.link_idx[li];
-            for (int part = 0; part < 2; ++part) {
-                const int beg = p.run_beg[2 * b2 + part];
-                const int len = p.run_len[2 * b2 + part];
-                total += len;
-                for (int j = lane; j < len; j += 64) {
-                    const int col = beg + j;
-                    const double t = dist2_lds(sq, p.feat + (int64_t)col * p.DP, p.DP);
-                    const int key = p.col_key[col];
-                    int pos;
-                    if (n < k) pos = n++;
-                    else if (less_tc(p, t, key, lt[(k - 1) * 64 + lane], lc[(k - 1) * 64 + lane])) pos = k - 1;
-                    else continue;
+        float4 qr[NV];
+        {
+            const float4* q4 = reinterpret_cast<const float4*>(p.feat + (int64_t)qc * p.DP);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) qr[i] = q4[i];
+        }
+        // batches: (run, offset) pairs walked in order; a batch is up to 64 rows of one run
+        const int l0 = p.link_ptr[c], l1 = p.link_ptr[c + 1];
+        int li = l0, part = 0, off = 0;
+        int beg = 0, len = 0;
+        auto next_run = [&]() {
+            // advance (li, part) to the next non-empty run at or after the current one
+            while (li < l1) {
+                const int b2 = s * p.C + p.link_idx[li];
+                len = p.run_len[2 * b2 + part];
+                beg = p.run_beg[2 * b2 + part];
+                if (len > 0) return true;
+                if (++part == 2) { part = 0; ++li; }
+            }
+            return false;
+        };
+        auto advance = [&]() {
+            off += 64;
+            if (off >= len) {
+                off = 0;
+                if (++part == 2) { part = 0; ++li; }
+                return next_run();
+            }
+            return true;
+        };
+        auto load = [&](int rbeg, int nrow) {
+            // rows rbeg .. rbeg + nrow - 1 (nrow <= 64) are contiguous; pieces past them re-read
+            // the batch's first row (a valid address) and are ignored
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int piece = i * 64 + lane;
+                const int row = piece / NV;
+                const float* src = p.feat + (int64_t)(rbeg + (row < nrow ? row : 0)) * p.DP + 4 * (piece - row * NV);
+                __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                                 (__attribute__((address_space(3))) void*)(stage + i * 64), 16, 0, 0);
+            }
+        };
+        __syncthreads();   // the previous query's LDS use is over
+        int n = 0, total = 0;
+        bool more = next_run();
+        int cbeg = beg + off, cn = len - off < 64 ? len - off : 64;
+        if (more) load(cbeg, cn);
+        while (more) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");     // the batch has landed in LDS
+            float4 b[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) b[i] = stage[lane * NV + i];
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // batch read before the next lands
+            const int bbeg = cbeg, bn = cn;
+            more = advance();
+            if (more) {
+                cbeg = beg + off;
+                cn = len - off < 64 ? len - off : 64;
+                load(cbeg, cn);
+            }
+            total += bn;
+            if (lane < bn) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i)
+                    asm volatile("" : "+v"(qr[i].x), "+v"(qr[i].y), "+v"(qr[i].z), "+v"(qr[i].w));
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    double d;
+                    d = (double)qr[i].x - (double)b[i].x; t = fma(d, d, t);
+                    d = (double)qr[i].y - (double)b[i].y; t = fma(d, d, t);
+                    d = (double)qr[i].z - (double)b[i].z; t = fma(d, d, t);
+                    d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
+                }
+                const int col = bbeg + lane;
+                const int key = p.col_key[col];
+                int pos = -1;
+                if (n < k) pos = n++;
+                else if (less_tc(p, t, key, lt[(k - 1) * 64 + lane], lc[(k - 1) * 64 + lane])) pos = k - 1;
+                if (pos >= 0) {
                     while (pos > 0 && less_tc(p, t, key, lt[(pos - 1) * 64 + lane], lc[(pos - 1) * 64 + lane])) {
                         lt[pos * 64 + lane] = lt[(pos - 1) * 64 + lane];
                         lc[pos * 64 + lane] = lc[(pos - 1) * 64 + lane];
@@ -112,16 +163,16 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
             wave_argmin(t, key, col, wl);
             if (wl == lane) ++head;
             if (lane == 0) {
-                knn_col[(int64_t)q * k + j] = col;
-                knn_t[(int64_t)q * k + j] = t;
+                knn_col[(int64_t)j * p.nq + q] = col;
+                knn_t[(int64_t)j * p.nq + q] = t;
             }
             last_t = t;
             last_c = col;
         }
         if (lane == 0) {
             for (int j = kk; j < k; ++j) {
-                knn_col[(int64_t)q * k + j] = last_c;
-                knn_t[(int64_t)q * k + j] = last_t;
+                knn_col[(int64_t)j * p.nq + q] = last_c;
+                knn_t[(int64_t)j * p.nq + q] = last_t;
             }
             dk2[q] = last_t;
             if (n_cand) n_cand[q] = total;
@@ -134,24 +185,38 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
 //   mode 0 (batch): rho_slot[slot] = float(1/sqrt(dk2)); dk_slot (opt) = float(sqrt(dk2));
 //                   knn64[slot*k+j] = col_ret (opt)
 //   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret
-// block (k, QPB): threadIdx.x = neighbour rank, threadIdx.y = query within the block, so the k
-// entries of a query are read and written as contiguous runs
-__global__ void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
-                             const double* __restrict__ dk2, float* __restrict__ rho_slot,
-                             float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
-                             int* __restrict__ knn32) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
-    if (i >= nq) return;
-    const int j = threadIdx.x;
-    const int slot = p.q_out[i];
-    if (j == 0) {
-        const double d = dk2[i];
+// A workgroup takes 64 queries: the slot-major lists are read with coalesced loads into an LDS
+// tile, then written out query-major (k contiguous entries per reported slot).
+constexpr int FIN_Q = 64;
+
+__global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
+                                                    const double* __restrict__ dk2, float* __restrict__ rho_slot,
+                                                    float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
+                                                    int* __restrict__ knn32) {
+    __shared__ int tile[IRC_KMAX * FIN_Q];
+    const int64_t i0 = (int64_t)blockIdx.x * FIN_Q;
+    const int nb = nq - i0 < FIN_Q ? (int)(nq - i0) : FIN_Q;
+    const int k = p.k;
+    const int t = threadIdx.x;
+    if (t < nb) {
+        const int slot = p.q_out[i0 + t];
+        const double d = dk2[i0 + t];
         if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
         if (dk_slot) dk_slot[slot] = (float)sqrt(d);
     }
-    const int v = p.col_ret[knn_col[i * p.k + j]];
-    if (knn64) knn64[(int64_t)slot * p.k + j] = v;
-    if (knn32) knn32[(int64_t)slot * p.k + j] = v;
+    if (!knn64 && !knn32) return;
+    for (int e = t; e < k * FIN_Q; e += blockDim.x) {
+        const int j = e / FIN_Q, c = e - j * FIN_Q;
+        if (c < nb) tile[j * FIN_Q + c] = p.col_ret[knn_col[(int64_t)j * p.nq + i0 + c]];
+    }
+    __syncthreads();
+    for (int e = t; e < k * nb; e += blockDim.x) {
+        const int c = e / k, j = e - c * k;
+        const int64_t slot = p.q_out[i0 + c];
+        const int v = tile[j * FIN_Q + c];
+        if (knn64) knn64[slot * k + j] = v;
+        if (knn32) knn32[slot * k + j] = v;
+    }
 }
 
 // rho per column from rho per slot (batch mode: slot = spike id - 1 = col_key - 1)
@@ -175,10 +240,10 @@ __global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_
     const float ra = rho_col[qc];
     const int k = p.k;
     for (int j = 0; j < k; ++j) {
-        const int col = knn_col[(int64_t)q * k + j];
+        const int col = knn_col[(int64_t)j * p.nq + q];
         if (rho_col[col] > ra) {
             const int slot = p.q_out[q];
-            const float dl = (float)(sqrt(knn_t[(int64_t)q * k + j]) * (double)ra);
+            const float dl = (float)(sqrt(knn_t[(int64_t)j * p.nq + q]) * (double)ra);
             if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
                 delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
                 nneigh_slot[slot] = p.col_ret[qc];
@@ -194,17 +259,19 @@ __global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_
 }
 
 // ---------------------------------------------------------------------------
-// Exact nearest denser neighbour: one wavefront per query, lanes over candidates.
+// Exact nearest denser neighbour: one wavefront per query, lanes over candidates.  Only denser
+// candidates (rho_b > rho_a, usually a few percent for the queries that reach this kernel) are
+// scored, so each lane loads just those rows; the query row is held in registers.
 //   mode 0 (batch): delta = float(sqrt(d2)*rho_a) with the sentinel rule; nneigh int64
 //   mode 1 (site) : dmin = float(sqrt(d2)) or +Inf; nneigh32 = col_ret (own when none)
 // The query list is qlist[0 .. *nq_dev) when nq_dev is set, else qlist[0..nq) / identity.
+template <int NV>
 __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __restrict__ qlist, int nq,
                                                     const int* __restrict__ nq_dev, int mode,
                                                     const float* __restrict__ rho_col,
                                                     float* __restrict__ delta_slot,
                                                     int64_t* __restrict__ nneigh64,
                                                     int* __restrict__ nneigh32) {
-    __shared__ double sq[IRC_DMAX];
     const int lane = threadIdx.x;
     const int n = nq_dev ? *nq_dev : nq;
     for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
@@ -213,9 +280,12 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
         const int blk = p.q_blk[q];
         const int s = blk / p.C, c = blk - s * p.C;
         const float ra = rho_col[qc];
-        __syncthreads();
-        if (lane < p.DP) sq[lane] = (double)p.feat[(int64_t)qc * p.DP + lane];
-        __syncthreads();
+        float4 qr[NV];
+        {
+            const float4* q4 = reinterpret_cast<const float4*>(p.feat + (int64_t)qc * p.DP);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) qr[i] = q4[i];
+        }
         double bt = __builtin_inf();
         int bk = 0x7fffffff, bc = -1;
         for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
@@ -226,7 +296,19 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
                 for (int j = lane; j < len; j += 64) {
                     const int col = beg + j;
                     if (!(rho_col[col] > ra)) continue;
-                    const double t = dist2_lds(sq, p.feat + (int64_t)col * p.DP, p.DP);
+                    const float4* x4 = reinterpret_cast<const float4*>(p.feat + (int64_t)col * p.DP);
+                    float4 b[NV];
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) b[i] = x4[i];
+                    double t = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NV; ++i) {
+                        double d;
+                        d = (double)qr[i].x - (double)b[i].x; t = fma(d, d, t);
+                        d = (double)qr[i].y - (double)b[i].y; t = fma(d, d, t);
+                        d = (double)qr[i].z - (double)b[i].z; t = fma(d, d, t);
+                        d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
+                    }
                     const int key = p.col_key[col];
                     if (bc < 0 || irc_less(t, key, bt, bk)) { bt = t; bk = key; bc = col; }
                 }
@@ -366,31 +448,44 @@ hipError_t launch_batch_queries(hipStream_t st, int64_t N, int S, int C, const i
     return hipGetLastError();
 }
 
-size_t knn_exact_lds_bytes(int k) { return sizeof(double) * IRC_DMAX + (size_t)k * 64 * (8 + 4); }
+size_t knn_exact_lds_bytes(int k, int DP) { return (size_t)64 * DP * 4 + (size_t)k * 64 * (8 + 4); }
+
+namespace {
+hipError_t knn_exact_go(hipStream_t st, const DevPlan& p, const int* qlist, int nq, const int* nq_dev, int grid,
+                        int* knn_col, double* knn_t, double* dk2, int* n_cand) {
+    const int g = grid < nq ? grid : nq;
+    const size_t lds = knn_exact_lds_bytes(p.k, p.DP);
+#define IRC_EX(NV)                                                                                           \
+    case NV:                                                                                                 \
+        k_knn_exact<NV><<<g, 64, lds, st>>>(p, qlist, nq, nq_dev, knn_col, knn_t, dk2, n_cand);              \
+        break;
+    switch (p.DP / 4) {
+        IRC_EX(1) IRC_EX(2) IRC_EX(3) IRC_EX(4) IRC_EX(5) IRC_EX(6) IRC_EX(7) IRC_EX(8)
+        IRC_EX(9) IRC_EX(10) IRC_EX(11) IRC_EX(12) IRC_EX(13) IRC_EX(14) IRC_EX(15) IRC_EX(16)
+        default: return hipErrorInvalidValue;
+    }
+#undef IRC_EX
+    return hipGetLastError();
+}
+}  // namespace
 
 hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, int grid,
                             int* knn_col, double* knn_t, double* dk2, int* n_cand) {
     if (nq <= 0) return hipSuccess;
-    const int g = grid < nq ? grid : nq;
-    k_knn_exact<<<g, 64, knn_exact_lds_bytes(p.k), st>>>(p, qlist, nq, nullptr, knn_col, knn_t, dk2, n_cand);
-    return hipGetLastError();
+    return knn_exact_go(st, p, qlist, nq, nullptr, grid, knn_col, knn_t, dk2, n_cand);
 }
 
 hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,
                                 int grid, int* knn_col, double* knn_t, double* dk2) {
     if (nq_max <= 0) return hipSuccess;
-    const int g = grid < nq_max ? grid : nq_max;
-    k_knn_exact<<<g, 64, knn_exact_lds_bytes(p.k), st>>>(p, qlist, nq_max, nq_dev, knn_col, knn_t, dk2, nullptr);
-    return hipGetLastError();
+    return knn_exact_go(st, p, qlist, nq_max, nq_dev, grid, knn_col, knn_t, dk2, nullptr);
 }
 
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
                              const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32) {
     if (nq <= 0) return hipSuccess;
-    const int qpb = 256 / p.k > 0 ? 256 / p.k : 1;
-    dim3 block(p.k, qpb);
-    k_knn_finish<<<(unsigned)((nq + qpb - 1) / qpb), block, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot,
-                                                                     knn64, knn32);
+    k_knn_finish<<<(unsigned)((nq + FIN_Q - 1) / FIN_Q), 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot,
+                                                                       knn64, knn32);
     return hipGetLastError();
 }
 
@@ -414,7 +509,16 @@ hipError_t launch_delta_exact(hipStream_t st, const DevPlan& p, const int* qlist
                               int* nneigh32) {
     if (nq <= 0) return hipSuccess;
     const int g = grid < nq ? grid : nq;
-    k_delta_exact<<<g, 64, 0, st>>>(p, qlist, nq, nq_dev, mode, rho_col, delta_slot, nneigh64, nneigh32);
+#define IRC_DX(NV)                                                                                        \
+    case NV:                                                                                              \
+        k_delta_exact<NV><<<g, 64, 0, st>>>(p, qlist, nq, nq_dev, mode, rho_col, delta_slot, nneigh64, nneigh32); \
+        break;
+    switch (p.DP / 4) {
+        IRC_DX(1) IRC_DX(2) IRC_DX(3) IRC_DX(4) IRC_DX(5) IRC_DX(6) IRC_DX(7) IRC_DX(8)
+        IRC_DX(9) IRC_DX(10) IRC_DX(11) IRC_DX(12) IRC_DX(13) IRC_DX(14) IRC_DX(15) IRC_DX(16)
+        default: return hipErrorInvalidValue;
+    }
+#undef IRC_DX
     return hipGetLastError();
 }
 

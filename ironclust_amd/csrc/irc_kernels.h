@@ -11,7 +11,7 @@ hipError_t launch_batch_queries(hipStream_t st, int64_t N, int S, int C, const i
                                 const int64_t* site_np, const int64_t* prim_ptr, const int64_t* lim,
                                 const int* col_spike, int64_t n_cols, int* q_col, int* q_blk, int* q_out,
                                 int* flags);
-size_t knn_exact_lds_bytes(int k);
+size_t knn_exact_lds_bytes(int k, int DP);
 hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, int grid,
                             int* knn_col, double* knn_t, double* dk2, int* n_cand);
 hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,

@@ -23,17 +23,19 @@ struct ScreenArgs {
     const void* qhi;         // [n_cols][KP] query-form rows [-2x, |x|^2, 1, 0...] (hi parts or fp32)
     const void* qlo;         // [n_cols][KP] query-form low parts (bf16 form)
     const float* norm;       // [n_cols] fp32 |x|^2
+    const float* run_nmax;   // [n_blocks * 2] largest |x|^2 of each (block, part) run
     const ScreenItem* items;
     int n_items;
-    int kprime;              // over-fetched candidates per query (32, 48 or 64)
+    int kprime;              // over-fetched candidates per query (32, 40, 48 or 64)
+    int large_ncand;         // blocks with more candidates keep 64 (when kprime > 32)
     int hist;                // run the histogram pre-pass that sets each query's starting threshold
     double eta;              // |screened - exact| <= eta * (|q|^2 + |c|^2)
     // certified candidate lists, slot-major ([slot][n_q]) so a wavefront's 64 queries write
     // and the re-rank's lanes read contiguous words
     int64_t n_q;             // queries of the plan (row stride of the lists)
     int* cand_n;             // per query: r | (min(k, |C|) << 16), or -1 (not screened / uncertified)
-    int* cand_col;           // [kprime][n_q] candidate columns in ascending screened order
-    float* cand_d;           // [kprime][n_q] their truncated screened distances
+    int* cand_col;           // [slots][n_q] candidate columns in ascending screened order
+    float* cand_d;           // [slots][n_q] their truncated screened distances
     double* cand_eps;        // per query: the screen's error bound
     int* fallback;           // queries to redo with the exact kernel
     int* n_fallback;
@@ -53,6 +55,9 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArg
 // ambiguous prefix does not fit the re-rank window join the fallback list.
 hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* knn_t, double* dk2);
 hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
+// largest |x|^2 of every (block, part) column run
+hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,
+                           float* out);
 hipError_t launch_mfma_layout_probe(hipStream_t st, float* out);
 
 }  // namespace irc

@@ -43,20 +43,41 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// One thread per (column, 8-dimension chunk): coalesced 16-byte stores of the candidate-form
-// rows [x, 1, |x|^2, 0...] and the query-form rows [-2x, |x|^2, 1, 0...] (bf16 hi/lo split, or fp32).
+// A workgroup packs 32 columns: their rows are staged in LDS with coalesced loads, one thread
+// per column forms |x|^2, then one thread per (column, 8-dimension chunk) writes coalesced
+// 16-byte pieces of the candidate-form rows [x, 1, |x|^2, 0...] and the query-form rows
+// [-2x, |x|^2, 1, 0...] (bf16 hi/lo split, or fp32).
+constexpr int PACK_COLS = 32;
+
 __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
                               void* __restrict__ hi, void* __restrict__ lo, void* __restrict__ qhi,
                               void* __restrict__ qlo, float* __restrict__ norm) {
+    __shared__ float srow[PACK_COLS * IRC_DMAX];
+    __shared__ float snorm[PACK_COLS];
     const int nch = KP / 8;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_cols * nch) return;
-    const int64_t col = e / nch;
-    const int j = (int)(e - col * nch);
-    const float* x = F + col * FS;
-    float n = 0.0f;
-    for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
-    if (j == 0) norm[col] = n;
+    const int64_t col0 = (int64_t)blockIdx.x * PACK_COLS;
+    const int ncol = n_cols - col0 < PACK_COLS ? (int)(n_cols - col0) : PACK_COLS;
+    // rows col0 .. col0+ncol-1 of F: (ncol-1)*FS + D contiguous floats
+    const int span = (ncol - 1) * FS + D;
+    const float* src = F + col0 * FS;
+    for (int e = threadIdx.x; e < span; e += blockDim.x) {
+        const int c = e / FS, d = e - c * FS;
+        if (d < D) srow[c * IRC_DMAX + d] = src[e];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncol) {
+        const float* x = srow + threadIdx.x * IRC_DMAX;
+        float n = 0.0f;
+        for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
+        snorm[threadIdx.x] = n;
+        norm[col0 + threadIdx.x] = n;
+    }
+    __syncthreads();
+    const int c = threadIdx.x / nch, j = threadIdx.x - c * nch;
+    if (c >= ncol) return;
+    const int64_t col = col0 + c;
+    const float* x = srow + c * IRC_DMAX;
+    const float n = snorm[c];
     float cv[8], qv[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
@@ -65,8 +86,7 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
         cv[t] = d < D ? xv : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
         qv[t] = d < D ? xv : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
     }
-    const int64_t off = col * KP + 8 * j;
-    if (prec == SCREEN_FP32) {
+    const int64_t off = col * KP + 8 * j;    if (prec == SCREEN_FP32) {
         float4* h4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(hi) + off);
         float4* q4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(qhi) + off);
         h4[0] = make_float4(cv[0], cv[1], cv[2], cv[3]);
@@ -98,16 +118,18 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
     }
 }
 
+// keys are signed: a negative screened distance (a candidate within the error bound of the
+// query) keeps its sign bit and sorts first
 template <int N>
-__device__ __forceinline__ void cex(uint32_t& a, uint32_t& b, bool up) {
-    const uint32_t lo = a < b ? a : b;
-    const uint32_t hi = a < b ? b : a;
+__device__ __forceinline__ void cex(int32_t& a, int32_t& b, bool up) {
+    const int32_t lo = a < b ? a : b;
+    const int32_t hi = a < b ? b : a;
     a = up ? lo : hi;
     b = up ? hi : lo;
 }
 
 template <int N>
-__device__ __forceinline__ void bitonic_sort(uint32_t* v) {
+__device__ __forceinline__ void bitonic_sort(int32_t* v) {
 #pragma unroll
     for (int k = 2; k <= N; k <<= 1) {
 #pragma unroll
@@ -122,7 +144,7 @@ __device__ __forceinline__ void bitonic_sort(uint32_t* v) {
 }
 
 template <int N>
-__device__ __forceinline__ void bitonic_clean(uint32_t* v) {
+__device__ __forceinline__ void bitonic_clean(int32_t* v) {
 #pragma unroll
     for (int j = N >> 1; j > 0; j >>= 1) {
 #pragma unroll
@@ -154,9 +176,16 @@ __device__ __forceinline__ double exact_d2(const float* __restrict__ q, const fl
     return acc;
 }
 
-__device__ __forceinline__ float key_dist(uint32_t key, uint32_t hmask) { return __uint_as_float(key & hmask); }
+// the screened distance a key stands for, negative values read as 0 (a valid lower bound, and
+// non-decreasing along a sorted list); the empty-slot key reads as NaN, which fails every test
+__device__ __forceinline__ float key_dist(int32_t key, uint32_t hmask) {
+    const float v = __uint_as_float((uint32_t)key & hmask);
+    return v < 0.0f ? 0.0f : v;
+}
+constexpr int32_t KEY_EMPTY = 0x7fffffff;
+constexpr int32_t KEY_NONE = (int32_t)0x80000000;   // threshold of lanes without a query
 
-constexpr int WIN = 8;  // re-rank window: longest run of mutually ambiguous neighbours
+constexpr int WIN = 16;  // re-rank window: longest run of mutually ambiguous neighbours
 
 template <int PREC, int KB, int LS, bool HIST>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
@@ -166,10 +195,11 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
     __shared__ int s_rb[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_ro[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_nr, s_item;
+    __shared__ float s_nmax;
     const DevPlan& p = a.p;
     const int lane = threadIdx.x;
     const int h = lane >> 5, m = lane & 31;
-    const int kprime = a.kprime;
+    const int kprime_base = a.kprime;
     // workgroups are dispatched round-robin over the 8 XCDs: blockIdx & 7 names the XCD
     const int my_seg = blockIdx.x & 7;
     for (;;) {
@@ -193,6 +223,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         const int s = item.blk / p.C, c = item.blk - s * p.C;
         if (lane == 0) {
             int nr = 0, off = 0;
+            float nm = 0.0f;
             for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
                 const int b2 = s * p.C + p.link_idx[li];
                 for (int part = 0; part < 2; ++part) {
@@ -201,16 +232,21 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                         s_rb[nr] = p.run_beg[2 * b2 + part];
                         s_ro[nr] = off;
                         off += len;
+                        nm = fmaxf(nm, a.run_nmax[2 * b2 + part]);
                         ++nr;
                     }
                 }
             }
             s_ro[nr] = off;
             s_nr = nr;
+            s_nmax = nm;
         }
         __syncthreads();
         const int nr = s_nr;
         const int ncand = item.ncand;
+        // long candidate streams keep a deeper list: their keys carry more position bits (coarser
+        // distances) and their neighbour distances are closer together
+        const int kprime = (LS == 64 && ncand > a.large_ncand) ? 64 : kprime_base;
 
         // ---- query operands (B) for the two 32-query column tiles (query-form rows)
         bf16x8 bh[2][KB], bl[2][KB];
@@ -244,35 +280,34 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         const int qc_mine = p.q_col[q_mine];
         const float nq_mine = a.norm[qc_mine];
 
-        uint32_t L[LS];
+        int32_t L[LS];
 #pragma unroll
-        for (int i = 0; i < LS; ++i) L[i] = 0xffffffffu;
-        uint32_t tau = qmine ? 0xffffffffu : 0u;
+        for (int i = 0; i < LS; ++i) L[i] = KEY_EMPTY;
+        int32_t tau = qmine ? KEY_EMPTY : KEY_NONE;
         int cnt = 0;
-        float nmax = 0.0f;
         const int ib = ceil_log2(ncand > 2 ? ncand : 2);
         const uint32_t pmask = (1u << ib) - 1u, hmask = ~pmask;
         int run = 0;
 
         // the lane's K'-th kept key: the append threshold (L may hold up to LS >= K' keys; those
         // past K' are larger than the threshold and never change the result)
-        auto kth = [&]() -> uint32_t {
+        auto kth = [&]() -> int32_t {
             if constexpr (LS == 32) return L[31];
             else return kprime == 40 ? L[39] : (kprime == 48 ? L[47] : (kprime == 32 ? L[31] : L[63]));
         };
-        uint32_t tau_init = tau;
+        int32_t tau_init = tau;
         // merge the oldest 32 buffered keys into L, move the rest (at most 32) down
         auto compact_half = [&]() {
-            uint32_t B[32];
+            int32_t B[32];
 #pragma unroll
             for (int i = 0; i < 32; ++i) {
-                const uint32_t v = sbuf[i * 64 + lane];
-                B[i] = i < cnt ? v : 0xffffffffu;
+                const int32_t v = (int32_t)sbuf[i * 64 + lane];
+                B[i] = i < cnt ? v : KEY_EMPTY;
             }
             bitonic_sort<32>(B);
 #pragma unroll
             for (int i = 0; i < 32; ++i) {
-                const uint32_t v = B[31 - i];
+                const int32_t v = B[31 - i];
                 L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
             }
             bitonic_clean<LS>(L);
@@ -281,29 +316,35 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = sbuf[(32 + i) * 64 + lane];
             }
             cnt = cnt > 32 ? cnt - 32 : 0;
-            const uint32_t t = kth();
-            tau = qmine ? (t < tau_init ? t : tau_init) : 0u;
+            const int32_t t = kth();
+            tau = qmine ? (t < tau_init ? t : tau_init) : KEY_NONE;
         };
 
         // operand rows of the lane's candidate at stream position pos (bf16: hi then lo halves)
         constexpr int NA = PREC == SCREEN_BF16 ? 2 * KB : 2 * KB;
         float4 An[NA];
-        float nnN = 0.0f;
-        // tile [pos0, pos0 + 32): `run` (wave-uniform) is the run holding pos0; the few runs that
-        // start inside the tile are walked with uniform branches, each lane keeping its column
+        // tile [pos0, pos0 + 32): `run` is the run holding pos0, with its start column, stream
+        // offset and the next run's offset kept wave-uniform (scalar registers); runs that start
+        // inside the tile are walked with uniform branches, each lane keeping its column
+        int run_rb = 0, run_ro = 0, run_next = 0;
+        auto run_enter = [&](int rr) {
+            run = rr;
+            run_rb = __builtin_amdgcn_readfirstlane(s_rb[rr]);
+            run_ro = __builtin_amdgcn_readfirstlane(s_ro[rr]);
+            run_next = rr + 1 < nr ? __builtin_amdgcn_readfirstlane(s_ro[rr + 1]) : 0x7fffffff;
+        };
         auto issue = [&](int pos0) {
-            while (run + 1 < nr && pos0 >= __builtin_amdgcn_readfirstlane(s_ro[run + 1])) ++run;
+            while (pos0 >= run_next) run_enter(run + 1);
             const int pos = pos0 + m;
-            int rr = run;
-            int col = __builtin_amdgcn_readfirstlane(s_rb[rr]) + (pos - __builtin_amdgcn_readfirstlane(s_ro[rr]));
-            while (rr + 1 < nr) {
-                const int ro = __builtin_amdgcn_readfirstlane(s_ro[rr + 1]);
-                if (ro >= pos0 + 32) break;
-                ++rr;
-                if (pos >= ro) col = __builtin_amdgcn_readfirstlane(s_rb[rr]) + (pos - ro);
+            int col = run_rb + (pos - run_ro);
+            if (run_next < pos0 + 32) {
+                for (int rr = run + 1; rr < nr; ++rr) {
+                    const int ro = __builtin_amdgcn_readfirstlane(s_ro[rr]);
+                    if (ro >= pos0 + 32) break;
+                    if (pos >= ro) col = __builtin_amdgcn_readfirstlane(s_rb[rr]) + (pos - ro);
+                }
             }
-            if (pos >= ncand) col = s_rb[0];
-            nnN = pos < ncand ? a.norm[col] : 0.0f;
+            if (pos >= ncand) col = run_rb;
             if constexpr (PREC == SCREEN_BF16) {
                 const float4* rh = reinterpret_cast<const float4*>(
                     reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)col * KP + 8 * h);
@@ -324,10 +365,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         // one pass over the candidate stream: MFMA distance tiles, then consume(v, cpos, valid)
         // for the 32 distances of the lane's query, then after_tile()
         auto stream = [&](auto&& consume, auto&& after_tile) {
-            run = 0;
+            run_enter(0);
             issue(0);
             for (int base = 0; base < ncand; base += 32) {
-                nmax = fmaxf(nmax, nnN);
                 const float4* Ac = An;
                 f32x16 acc0 = {}, acc1 = {};
                 if constexpr (PREC == SCREEN_BF16) {
@@ -404,18 +444,17 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                     cum += (int)sbuf[b * 64 + lane];
                     if (cum >= kprime) { tb = b; break; }
                 }
-                if (qmine && tb < 63) tau = (uint32_t)(hbase + ((tb + 1) << 21));
+                if (qmine && tb < 63) tau = hbase + ((tb + 1) << 21);
                 tau_init = tau;
                 __syncthreads();
             }
         }
-        nmax = 0.0f;
 
-        // ---- pass 2: keys (non-negative distance bits above the candidate's stream position)
-        // below the threshold go to the lane's append buffer; full buffers merge into L
+        // ---- pass 2: keys (the screened distance's bits, sign kept, above the candidate's stream
+        // position) below the threshold go to the lane's append buffer; full buffers merge into L
         stream(
             [&](int sbits, int cpos, bool valid) {
-                const uint32_t key = ((uint32_t)max(sbits, 0) & hmask) | (uint32_t)cpos;
+                const int32_t key = (int32_t)(((uint32_t)sbits & hmask) | (uint32_t)cpos);
                 // slot cnt (< 64: at most 32 buffered when a tile starts) is scratch until counted
                 sbuf[cnt * 64 + lane] = key;
                 cnt += (key < tau && valid) ? 1 : 0;
@@ -425,13 +464,11 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             });
         while (__any(cnt > 0)) compact_half();
 
-        // ---- finalize: certification, FP64 re-rank of the ambiguous prefix, output
-        float nmax_w = nmax;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) nmax_w = fmaxf(nmax_w, __shfl_xor(nmax_w, off, 64));
+        // ---- finalize: certification and the list of candidates the FP64 re-rank must see
+        const float nmax_w = s_nmax;   // largest |c|^2 of the block's candidates
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < LS; ++i) sbuf[i * 64 + lane] = L[i];
+        for (int i = 0; i < LS; ++i) sbuf[i * 64 + lane] = (uint32_t)L[i];
         __syncthreads();
         if (qmine) {
             const int k = p.k;
@@ -439,18 +476,19 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const int kk = ncand < k ? ncand : k;
             const double eps = a.eta * ((double)nq_mine + (double)nmax_w) + 1e-30;
             const double qf = ldexp(1.0, -(22 - ib));
-            const double dkt = key_dist(sbuf[(kk - 1) * 64 + lane], hmask);
+            const double dkt = key_dist((int32_t)sbuf[(kk - 1) * 64 + lane], hmask);
             // every one of the first kk kept candidates has d^2 <= U
             const double U = dkt * (1.0 + qf) + eps;
             // every candidate outside L has a key at least min(L[K'-1], starting threshold)
-            const uint32_t kexcl = sbuf[(kprime - 1) * 64 + lane] < tau_init ? sbuf[(kprime - 1) * 64 + lane] : tau_init;
+            const int32_t kl = (int32_t)sbuf[(kprime - 1) * 64 + lane];
+            const int32_t kexcl = kl < tau_init ? kl : tau_init;
             const bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
             if (ok) {
                 // the re-rank needs every kept candidate whose lower bound reaches U
                 int r = kk;
-                while (r < nl && (double)key_dist(sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
+                while (r < nl && (double)key_dist((int32_t)sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
                 for (int j = 0; j < r; ++j) {
-                    const uint32_t key = sbuf[j * 64 + lane];
+                    const int32_t key = (int32_t)sbuf[j * 64 + lane];
                     const int pos = (int)(key & pmask);
                     int lo = 0, hi = nr - 1;
                     while (lo < hi) {
@@ -471,50 +509,26 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
     }
 }
 
-// FP64 re-rank of the certified lists.  A workgroup takes 64 consecutive queries (one
-// (site, chunk) block, or two: their candidate rows are shared and stay in L2).
-// Phase 1: the four wavefronts split the list slots, each lane computing the canonical FP64
-// distance of one (query, slot) pair into LDS -- independent row gathers, many in flight.
-// Phase 2: lane l of wavefront 0 walks query l's slots in ascending screened order with
-// lower bounds lb_j = d_j - eps; a window of WIN exact distances kept in (d^2, id) order emits
-// its head once the head is below the next slot's lower bound (no later slot can precede it).
-// A window overflow (more than WIN mutually ambiguous candidates) sends the query to the exact
-// kernel.
-constexpr int RR_WAVES = 4;
-
-// canonical d^2 of a query row held in registers and a DP-padded candidate row; NV = DP / 4
-// is known at compile time so every row load is issued before the FMA chain starts
+// FP64 re-rank of the certified lists.  One wavefront takes 64 consecutive queries (lane l =
+// query l) and walks the list slots in ascending screened order.  For each slot the 64
+// candidate rows are gathered cooperatively -- consecutive lanes fetch consecutive 16-byte
+// pieces of a row, straight into LDS (global_load_lds), so a load instruction touches a few
+// rows instead of 64 -- and each lane then reads its own row back and runs the canonical FP64
+// chain against its query row, held in registers.  The next slot's gather is in flight while
+// the chain runs.  Each lane keeps a window of WIN exact distances in (d^2, id) order and emits
+// its head once the head is below the next slot's lower bound d_j - eps (no later slot can
+// precede it).  A window overflow (more than WIN mutually ambiguous candidates) sends the
+// query to the exact kernel.
 template <int NV>
-__device__ __forceinline__ double exact_d2_n(const float4 (&a)[NV], const float* __restrict__ x) {
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    float4 b[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) b[i] = x4[i];
-    double acc = 0.0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double t;
-        t = (double)a[i].x - (double)b[i].x; acc = fma(t, t, acc);
-        t = (double)a[i].y - (double)b[i].y; acc = fma(t, t, acc);
-        t = (double)a[i].z - (double)b[i].z; acc = fma(t, t, acc);
-        t = (double)a[i].w - (double)b[i].w; acc = fma(t, t, acc);
-    }
-    return acc;
-}
-
-template <int NV>
-__global__ __launch_bounds__(64 * RR_WAVES) void k_rerank(DevPlan p, int64_t n_q, int kslots,
-                                                         const int* __restrict__ cand_n,
-                                                         const int* __restrict__ cand_col,
-                                                         const float* __restrict__ cand_d,
-                                                         const double* __restrict__ cand_eps,
-                                                         int* __restrict__ knn_col, double* __restrict__ knn_t,
-                                                         double* __restrict__ dk2, int* __restrict__ fallback,
-                                                         int* __restrict__ n_fallback) {
-    extern __shared__ double rr_smem[];
-    double* sT = rr_smem;                                          // [kslots][64]
-    int* sK = reinterpret_cast<int*>(rr_smem + (size_t)kslots * 64); // [kslots][64] spike keys
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+__global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int* __restrict__ cand_n,
+                                              const int* __restrict__ cand_col, const float* __restrict__ cand_d,
+                                              const double* __restrict__ cand_eps, int* __restrict__ knn_col,
+                                              double* __restrict__ knn_t, double* __restrict__ dk2,
+                                              int* __restrict__ fallback, int* __restrict__ n_fallback) {
+    __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
+    __shared__ int skey[64];            // their spike ids
+    __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
+    const int l = threadIdx.x;
     // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
     // of the query groups, so the candidate rows its groups share stay in its L2
     const unsigned G = gridDim.x, x = blockIdx.x & 7, per = (G + 7) >> 3, lo8 = G & 7;
@@ -524,70 +538,134 @@ __global__ __launch_bounds__(64 * RR_WAVES) void k_rerank(DevPlan p, int64_t n_q
     const bool qv = q < n_q;
     const int rn = qv ? cand_n[q] : -1;
     const int r = rn < 0 ? 0 : (rn & 0xffff);
-    if (qv && r > w) {
-        const float4* q4 = reinterpret_cast<const float4*>(p.feat + (int64_t)p.q_col[q] * p.DP);
-        float4 qr[NV];
+    int rmax = r;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off, 64));
+    if (rmax == 0) return;
+    const int qc = p.q_col[qv ? q : 0];
+    float4 qr[NV];
+    {
+        const float4* q4 = reinterpret_cast<const float4*>(p.feat + (int64_t)qc * p.DP);
 #pragma unroll
         for (int i = 0; i < NV; ++i) qr[i] = q4[i];
-        for (int j = w; j < r; j += RR_WAVES) {
-            const int col = cand_col[(int64_t)j * n_q + q];
-            sT[j * 64 + l] = exact_d2_n<NV>(qr, p.feat + (int64_t)col * p.DP);
-            sK[j * 64 + l] = p.col_key[col];
-        }
     }
-    __syncthreads();
-    if (w != 0 || !qv || rn < 0) return;
-    const int kk = rn >> 16;
+    // lanes past their own list fetch their query's row (a valid address) and ignore it
+    auto slot_col = [&](int j) { return j < r ? cand_col[(int64_t)j * n_q + q] : qc; };
+    // slot j's rows, ids and the next slot's screened distances, straight into LDS
+    auto gather = [&](int j, int colj) {
+        const int jn = j + 1 < r ? j + 1 : (r > 0 ? r - 1 : 0);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(p.col_key + colj),
+                                         (__attribute__((address_space(3))) void*)skey, 4, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(cand_d + (int64_t)jn * n_q + (qv ? q : 0)),
+            (__attribute__((address_space(3))) void*)sdn, 4, 0, 0);
+        int crow[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) crow[i] = __shfl(colj, (i * 64 + l) / NV, 64);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int piece = i * 64 + l;
+            const int part = piece - (piece / NV) * NV;
+            const float* src = p.feat + (int64_t)crow[i] * p.DP + 4 * part;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(stage + i * 64), 16, 0, 0);
+        }
+    };
     const int k = p.k;
-    const double eps = cand_eps[q];
+    const int kk = rn >> 16;
+    const double eps = qv && rn >= 0 ? cand_eps[q] : 0.0;
     double wt[WIN];
     int wk[WIN], wc[WIN];
     int wn = 0, out = 0;
     double last_t = 0.0;
     int last_c = -1;
-    bool ok = true;
-    for (int j = 0; j < r && out < kk; ++j) {
-        const double t = sT[j * 64 + l];
-        const int kc = sK[j * 64 + l];
-        const int col = cand_col[(int64_t)j * n_q + q];
-        const double lb = j + 1 < r ? (double)cand_d[(int64_t)(j + 1) * n_q + q] - eps : __builtin_inf();
-        if (wn == WIN) { ok = false; break; }
-        bool placed = false;
+    bool ok = rn >= 0;
+    int colj = slot_col(0);
+    int coln = slot_col(1);
+    gather(0, colj);
+    for (int j = 0; j < rmax; ++j) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the gather of slot j has landed in LDS
+        float4 b[NV];
 #pragma unroll
-        for (int i = WIN - 1; i >= 0; --i) {
-            if (i <= wn && !placed) {
-                if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
-                    wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
-                } else {
-                    wt[i] = t; wk[i] = kc; wc[i] = col;
-                    placed = true;
+        for (int i = 0; i < NV; ++i) b[i] = stage[l * NV + i];
+        const int kc = skey[l];
+        const float dn = sdn[l];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // LDS read before the next gather lands
+        const bool act = ok && j < r && out < kk;
+        const int coln2 = slot_col(j + 2);
+        if (j + 1 < rmax) gather(j + 1, coln);
+        if (act) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+                asm volatile("" : "+v"(qr[i].x), "+v"(qr[i].y), "+v"(qr[i].z), "+v"(qr[i].w));
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double d;
+                d = (double)qr[i].x - (double)b[i].x; t = fma(d, d, t);
+                d = (double)qr[i].y - (double)b[i].y; t = fma(d, d, t);
+                d = (double)qr[i].z - (double)b[i].z; t = fma(d, d, t);
+                d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
+            }
+            const double lb = j + 1 < r ? (double)dn - eps : __builtin_inf();
+            if (wn == WIN) {
+                ok = false;
+            } else {
+                bool placed = false;
+#pragma unroll
+                for (int i = WIN - 1; i >= 0; --i) {
+                    if (i <= wn && !placed) {
+                        if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
+                            wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
+                        } else {
+                            wt[i] = t; wk[i] = kc; wc[i] = colj;
+                            placed = true;
+                        }
+                    }
+                }
+                ++wn;
+                while (wn > 0 && out < kk && wt[0] < lb) {
+                    knn_col[(int64_t)out * n_q + q] = wc[0];
+                    knn_t[(int64_t)out * n_q + q] = wt[0];
+                    last_t = wt[0];
+                    last_c = wc[0];
+                    ++out;
+#pragma unroll
+                    for (int i = 0; i < WIN - 1; ++i) {
+                        if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
+                    }
+                    --wn;
                 }
             }
         }
-        ++wn;
-        while (wn > 0 && out < kk && wt[0] < lb) {
-            knn_col[q * k + out] = wc[0];
-            knn_t[q * k + out] = wt[0];
-            last_t = wt[0];
-            last_c = wc[0];
-            ++out;
-#pragma unroll
-            for (int i = 0; i < WIN - 1; ++i) {
-                if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
-            }
-            --wn;
-        }
+        colj = coln;
+        coln = coln2;
     }
+    if (!qv || rn < 0) return;
     if (ok && out == kk) {
         for (int j = kk; j < k; ++j) {
-            knn_col[q * k + j] = last_c;
-            knn_t[q * k + j] = last_t;
+            knn_col[(int64_t)j * n_q + q] = last_c;
+            knn_t[(int64_t)j * n_q + q] = last_t;
         }
         dk2[q] = last_t;
     } else {
         const int f = atomicAdd(n_fallback, 1);
         fallback[f] = (int)q;
     }
+}
+
+// one wavefront per run
+__global__ void k_run_nmax(int64_t n_runs, const int* __restrict__ run_beg, const int* __restrict__ run_len,
+                           const float* __restrict__ norm, float* __restrict__ out) {
+    const int64_t rr = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int l = threadIdx.x & 63;
+    if (rr >= n_runs) return;
+    const int beg = run_beg[rr], len = run_len[rr];
+    float m = 0.0f;
+    for (int i = l; i < len; i += 64) m = fmaxf(m, norm[beg + i]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (l == 0) out[rr] = m;
 }
 
 __global__ void k_fill_int(int* p, int64_t n, int v) {
@@ -666,9 +744,9 @@ double screen_eta(int prec, int D, int KP) {
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
                               void* hi, void* lo, void* qhi, void* qlo, float* norm) {
     if (n_cols <= 0) return hipSuccess;
-    const int64_t total = n_cols * (KP / 8);
-    k_screen_pack<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
-                                                                    norm);
+    const int64_t blocks = (n_cols + PACK_COLS - 1) / PACK_COLS;
+    k_screen_pack<<<(unsigned)blocks, PACK_COLS * (KP / 8), 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
+                                                                     norm);
     return hipGetLastError();
 }
 
@@ -697,13 +775,11 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArg
 
 hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* knn_t, double* dk2) {
     if (a.n_q <= 0) return hipSuccess;
-    const size_t smem = (size_t)a.kprime * 64 * (sizeof(double) + sizeof(int));
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
-#define IRC_RR(NV)                                                                                        \
-    case NV:                                                                                              \
-        k_rerank<NV><<<grid, 64 * RR_WAVES, smem, st>>>(a.p, a.n_q, a.kprime, a.cand_n, a.cand_col, a.cand_d, \
-                                                        a.cand_eps, knn_col, knn_t, dk2, a.fallback,      \
-                                                        a.n_fallback);                                    \
+#define IRC_RR(NV)                                                                                         \
+    case NV:                                                                                               \
+        k_rerank<NV><<<grid, 64, 0, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, knn_t, \
+                                          dk2, a.fallback, a.n_fallback);                                  \
         break;
     switch (a.p.DP / 4) {
         IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)
@@ -711,6 +787,13 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
         default: return hipErrorInvalidValue;
     }
 #undef IRC_RR
+    return hipGetLastError();
+}
+
+hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,
+                           float* out) {
+    if (n_runs <= 0) return hipSuccess;
+    k_run_nmax<<<(unsigned)((n_runs + 3) / 4), 256, 0, st>>>(n_runs, run_beg, run_len, norm, out);
     return hipGetLastError();
 }
 

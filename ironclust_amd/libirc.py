@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_PKG, "_lib", "libirc_hip.so")
 IRC_OK, IRC_E_ARG, IRC_E_NOMEM, IRC_E_HIP, IRC_E_UNSUPPORTED, IRC_E_FORCED = range(6)
 PREC_EXACT, PREC_FP32, PREC_BF16_SCREEN, PREC_AUTO = range(4)
 MEM_HOST, MEM_DEVICE = 0, 1
+FLAG_HIST, FLAG_GRAPH = 2, 4          # irc_options.flags
 
 EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
@@ -30,7 +31,8 @@ _c_u8p = ctypes.POINTER(ctypes.c_uint8)
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("k_over", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("force_fail", ctypes.c_int32), ("mem", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("force_fail", ctypes.c_int32), ("mem", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("large_ncand", ctypes.c_int32)]
 
 
 class SiteProblem(ctypes.Structure):
@@ -56,7 +58,8 @@ class Stats(ctypes.Structure):
                 ("n_screen_fallback", ctypes.c_int64), ("n_delta_full", ctypes.c_int64),
                 ("ms_total", ctypes.c_double), ("ms_knn", ctypes.c_double), ("ms_delta", ctypes.c_double),
                 ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32),
-                ("ms_rerank", ctypes.c_double)]
+                ("ms_rerank", ctypes.c_double), ("n_rerank_overflow", ctypes.c_int64),
+                ("ms_graph", ctypes.c_double)]
 
 
 class IrcError(RuntimeError):
@@ -131,7 +134,8 @@ def _dptr(t, ct):
     return ctypes.cast(ctypes.c_void_p(t.data_ptr()), ct)
 
 
-def make_options(precision=PREC_AUTO, k_over=0, force_fail=False, mem=MEM_HOST, flags=0):
+def make_options(precision=PREC_AUTO, k_over=0, force_fail=False, mem=MEM_HOST, flags=0, tile=0,
+                 waves_per_cu=0, large_ncand=0):
     lib = load()
     o = Options()
     lib.irc_options_default(ctypes.byref(o))
@@ -140,6 +144,9 @@ def make_options(precision=PREC_AUTO, k_over=0, force_fail=False, mem=MEM_HOST, 
     o.force_fail = 1 if force_fail else 0
     o.mem = mem
     o.flags = flags
+    o.tile = tile
+    o.waves_per_cu = waves_per_cu
+    o.large_ncand = large_ncand
     return o
 
 

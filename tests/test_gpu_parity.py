@@ -105,8 +105,54 @@ def test_full_size_site_subset_bf16():
     assert bool((out["rho"][out["nneigh"][has] - 1] > out["rho"][has]).all())
 
 
+def test_graph_replay_matches_eager():
+    """IRC_FLAG_GRAPH: the first call runs and captures the device phase, later calls replay it.
+    Every replay (with unrelated eager work and synchronizations in between) must equal the eager
+    call, also after the features change in place."""
+    import torch
+    ds = synth.make_dataset(24000, 4, 16, 8, 5, k=12, drift_step=0.3, n_links=3, band=2, seed=39, f2=True,
+                            device="cuda")
+    lay = irc2.site_layout(ds)
+    N, k = ds["n_spikes"], ds["k"]
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"], n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+    ctx = libirc.Context(0)
+    try:
+        for feats_round in range(2):
+            torch.cuda.synchronize()
+            ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
+            ref = {kk: v.clone() for kk, v in out.items()}
+            for rep in range(4):
+                for v in out.values():
+                    v.zero_()
+                junk = torch.randn(1 << 20, device="cuda") * 2.0          # eager work between replays
+                torch.cuda.synchronize()
+                ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN,
+                                                                         flags=libirc.FLAG_GRAPH))
+                st = ctx.stats()
+                for kk in ref:
+                    assert torch.equal(out[kk], ref[kk]), (feats_round, rep, kk)
+                if rep > 0 or feats_round > 0:
+                    assert st["ms_graph"] > 0.0, st
+                del junk
+            if feats_round == 0:
+                # the features change in place: the replays must read the new values
+                torch.cuda.synchronize()
+                dev["F"].mul_(1.25)
+        r = O.sort_batch(dict(ds, F1=ds["F1"].cpu() * 1.25, F2=ds["F2"].cpu() * 1.25,
+                              site1=ds["site1"].cpu(), site2=ds["site2"].cpu()))
+        np.testing.assert_array_equal(out["rho"].cpu().numpy(), r["rho"])
+        np.testing.assert_array_equal(out["knn"].cpu().numpy(), r["knn"])
+    finally:
+        ctx.close()
+
+
 def test_batch_odd_dims_and_k():
-    for D, k in ((1, 1), (5, 3), (45, 33), (62, 64)):
+    for D, k in ((1, 1), (5, 3), (45, 33), (62, 64), (64, 7)):
         ds = synth.make_dataset(3000, 2, D, 3, 3, k=k, drift_step=0.1, n_links=2, band=1, seed=36 + D)
         for prec in PRECS:
             _check_batch(ds, prec, f"D{D}k{k}")
