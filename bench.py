@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--autotune", action="store_true",
                     help="pick k_over / grid / list depth / pre-pass / group size per form by timing them first "
                          "(outside the timed steps)")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time replays of the call's device phase captured in a HIP graph")
     ap.add_argument("--per-site", type=int, default=0,
                     help="also time the per-site call pattern (irc_knn_drift + irc_delta_drift per site, host "
                          "buffers, the reference's rho_paged_site_/delta_paged_site_ loop) on this many sites")
@@ -113,10 +115,23 @@ def main():
             "delta_full_scan": int(s["n_delta_full"]),
             "precision_used": int(s["precision_used"]),
         }
-        if len(a.forms.split(",")) > 1:
+        if len(a.forms.split(",")) > 1 or a.graph:
             results[form] = {kk: v.clone() for kk, v in out.items()}
         else:
             results[form] = out
+        if a.graph:
+            go = libirc.make_options(precision=PREC[form], **dict(tuned.get(form, {}).get("options", {}),
+                                                                   flags=libirc.FLAG_GRAPH |
+                                                                   tuned.get(form, {}).get("options", {}).get("flags", 0)))
+            ctx.rho_delta_batch_device(dev, out, go)          # runs once and records the graph
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(a.steps):
+                ctx.rho_delta_batch_device(dev, out, go)
+            torch.cuda.synchronize()
+            forms[form]["graph_replay_ms_per_step"] = (time.perf_counter() - t2) / a.steps * 1e3
+            forms[form]["graph_ms_device"] = ctx.stats()["ms_graph"]
+            forms[form]["graph_identical"] = all(torch.equal(out[kk], results[form][kk]) for kk in out)
         print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
     head = a.forms.split(",")[0]
     # all forms must agree bit for bit

@@ -226,6 +226,34 @@ irc_status irc_delta_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, 
                               const float* fet1, const float* fet2, const float* rho12, float dc2,
                               float* delta_out, uint32_t* nneigh_out);
 
+/*
+ * Drift linkage: calc_drift_ (matlab/irc2.m:6373-6447) with dist_mask_ml_ (6452-6462),
+ * quantile_vr_ (6478-6486), subsample_vr_ (1557-1572) and mi2ml_drift_ (6508-6515).
+ * Produces the viLim_drift / mlDrift pair that the kNN and delta entry points consume.
+ *   amp, pos    n_spikes single (vrAmp_spk, mrPos_spk(:,2)), host memory
+ *   n_amp       nQuantile_drift (amplitude quantile bins), n_pos = numel(viSite2Chan)
+ *   n_drift     nTime_drift (chunks), n_batch = nTime_batch (links per chunk),
+ *   n_time_max  nTime_max_drift: above it, chunk c may link only to chunks within
+ *               n_time_max/4 (plus the corner blocks of calc_drift_)
+ * Outputs: *n_drift_out = C (1 when n_batch >= n_drift: one chunk, linked to itself);
+ * lim_out[C+1] (1-based, viLim_drift); link_out[C*C] column-major, link_out[r + c*C] =
+ * mlDrift(r, c); counts_out (optional) [(n_amp*n_pos) x C] int32 histograms (mrCount_drift).
+ * Ranking is by exact squared distance between count vectors, ties to the lower chunk.
+ */
+typedef struct {
+    int64_t n_spikes;
+    const float* amp;
+    const float* pos;
+    int32_t n_amp;
+    int32_t n_pos;
+    int32_t n_drift;
+    int32_t n_batch;
+    int32_t n_time_max;
+} irc_drift_problem;
+
+irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_drift_out, int64_t* lim_out,
+                          uint8_t* link_out, int32_t* counts_out);
+
 #ifdef __cplusplus
 }
 #endif

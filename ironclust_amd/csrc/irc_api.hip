@@ -10,12 +10,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "irc_common.h"
 #include "irc_kernels.h"
+#include "irc_drift.h"
 #include "irc_plan.h"
 #include "irc_screen.h"
 #include "../../include/irc_hip.h"
@@ -46,6 +48,7 @@ struct irc_ctx {
     DevBuf b_scr[8];
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
     DevBuf b_plan;                                         // device-side work planning
+    DevBuf b_drift;                                        // drift linkage scratch
     DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
                           // re-rank, [2] pairs, [3] queries whose denser neighbour was outside the kNN list
     bool dstat_screen = false;  // [0], [1] hold this call's screen counters
@@ -332,7 +335,7 @@ struct DevWork {
     int S, C;
     const int64_t* site_ptr;   // device [S+1]
     const int64_t* prim;       // device [S+1] prefix of primary counts
-    int64_t max_site_cols;     // largest column count of a site: bounds every block's candidates
+    int64_t max_ncand;         // largest candidate count of a block
 };
 
 // kNN pass: MFMA screen + certification + FP64 re-rank for every block that fits the
@@ -436,7 +439,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
         pb = irc::plan_buf_carve(ctx->b_plan.p, nb);
         HIPCHK(irc::launch_plan(st, dw->S, dw->C, p, dw->site_ptr, dw->prim, min_item, IRC_SCREEN_MAXCAND,
                                 IRC_SCREEN_MAXRUNS, pb, ptr<irc::ScreenItem>(ctx->b_scr[3]), fb, d_seg_beg, nfb));
-        any_large = dw->max_site_cols > large_ncand;
+        any_large = dw->max_ncand > large_ncand;
     }
     const int KP = irc::screen_kp(p.D);
     const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
@@ -510,6 +513,52 @@ irc_status collect_stats(irc_ctx* ctx, bool events) {
     return IRC_OK;
 }
 
+// MATLAB round(): half away from zero
+double round_half_away(double x) { return x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5); }
+
+// subsample_vr_ (matlab/irc2.m:1557-1572) then quantile_vr_ (6478-6486): the sorted subsample at
+// max(min(round(n q), n), 1) for q = (0:nq)/nq
+std::vector<float> drift_quantiles(const float* v, int64_t n, int nq) {
+    const int64_t n_max = 100000;
+    std::vector<float> s;
+    if (n <= n_max) {
+        s.assign(v, v + n);
+    } else {
+        const int64_t skip = n / n_max;
+        if (skip > 1) {
+            for (int64_t i = 0; i < n; i += skip) s.push_back(v[i]);
+        } else {
+            s.assign(v, v + n);
+        }
+        if ((int64_t)s.size() > n_max) {
+            const int64_t m = (int64_t)s.size(), n_remove = m - n_max;
+            std::vector<char> keep((size_t)m, 1);
+            for (int64_t j = 0; j < n_remove; ++j) {
+                // linspace(1, m, n_remove)
+                const double x = n_remove == 1 ? (double)m : 1.0 + (double)(m - 1) * (double)j / (double)(n_remove - 1);
+                int64_t r = (int64_t)round_half_away(x);
+                r = r < 1 ? 1 : (r > m ? m : r);
+                keep[(size_t)(r - 1)] = 0;
+            }
+            std::vector<float> t;
+            t.reserve((size_t)n_max);
+            for (int64_t i = 0; i < m; ++i)
+                if (keep[(size_t)i]) t.push_back(s[(size_t)i]);
+            s.swap(t);
+        }
+    }
+    std::stable_sort(s.begin(), s.end());
+    const int64_t m = (int64_t)s.size();
+    std::vector<float> q((size_t)nq + 1);
+    for (int i = 0; i <= nq; ++i) {
+        int64_t idx = (int64_t)round_half_away((double)m * ((double)i / (double)nq));
+        idx = idx > m ? m : idx;
+        idx = idx < 1 ? 1 : idx;
+        q[(size_t)i] = s[(size_t)(idx - 1)];
+    }
+    return q;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -563,7 +612,8 @@ void irc_ctx_destroy(irc_ctx* ctx) {
                      &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_knnt, &ctx->b_dk2, &ctx->b_rhoslot,
                      &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
                      &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
-                     &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan};
+                     &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan, &ctx->b_dstat,
+                     &ctx->b_drift};
     for (DevBuf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->b_scr)
@@ -798,16 +848,18 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
 
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = N;
-    HIPCHK(irc::launch_fill_int(st, d_flags, 2, 0));
+    HIPCHK(irc::launch_fill_int(st, d_flags, 3, 0));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     HIPCHK(irc::launch_pack_fp32(st, n_cols, D, DP, dF, ptr<float>(ctx->b_feat)));
     HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
                                   ptr<int>(ctx->b_runl)));
     HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
                                      ptr<int>(ctx->b_qcol), ptr<int>(ctx->b_qblk), ptr<int>(ctx->b_qout), d_flags));
-    int flags = 0;
-    HIPCHK(hipMemcpyAsync(&flags, d_flags, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(irc::launch_max_ncand(st, S, C, p.link_ptr, p.link_idx, p.run_len, d_flags + 2));
+    int hflags[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(hflags, d_flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    const int flags = hflags[0];
     if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
     if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
     if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
@@ -817,8 +869,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     dw.C = C;
     dw.site_ptr = d_site_ptr;
     dw.prim = d_prim;
-    dw.max_site_cols = 0;
-    for (int s = 0; s < S; ++s) dw.max_site_cols = std::max<int64_t>(dw.max_site_cols, b->site_ptr[s + 1] - b->site_ptr[s]);
+    dw.max_ncand = hflags[2];
     // the device phase: kNN (plan, pack, screen, re-rank, exact), rho, delta.  With
     // IRC_FLAG_GRAPH it is captured once into a HIP graph and replayed while the call's
     // descriptor (pointers, sizes, options, host descriptor contents) is unchanged.
@@ -1032,6 +1083,69 @@ irc_status irc_delta_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, 
     HIPCHK(hipMemcpyAsync(delta_out, ctx->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(nneigh_out, ctx->b_out_nn32.p, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
+irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_drift_out, int64_t* lim_out,
+                          uint8_t* link_out, int32_t* counts_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (!pr || !pr->amp || !pr->pos || !n_drift_out || !lim_out || !link_out)
+        return fail(ctx, IRC_E_ARG, "null pointer in irc_calc_drift");
+    const int64_t N = pr->n_spikes;
+    const int C = pr->n_drift, na = pr->n_amp, np = pr->n_pos;
+    if (N <= 0 || C <= 0 || na <= 0 || np <= 0 || pr->n_batch <= 0 || pr->n_time_max < 4)
+        return fail(ctx, IRC_E_ARG, "invalid sizes in irc_calc_drift");
+    // one chunk's histogram and the bin edges live in LDS (64 KB)
+    if (((int64_t)na * np + na + np + 2) * 4 > 65536)
+        return fail(ctx, IRC_E_UNSUPPORTED, "n_amp * n_pos too large (%d x %d)", na, np);
+    if (pr->n_batch >= C) {
+        // one chunk linked to itself (matlab/irc2.m:6388-6391)
+        *n_drift_out = 1;
+        lim_out[0] = 1;
+        lim_out[1] = N + 1;
+        link_out[0] = 1;
+        return IRC_OK;
+    }
+    if (C > 16384) return fail(ctx, IRC_E_UNSUPPORTED, "n_drift above 16384");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    std::vector<int64_t> lim((size_t)C + 1);
+    lim[0] = 1;
+    for (int c = 1; c <= C; ++c) lim[(size_t)c] = (int64_t)std::ceil((double)c / C * (double)N) + 1;
+    const std::vector<float> qa = drift_quantiles(pr->amp, N, na);
+    const std::vector<float> qp = drift_quantiles(pr->pos, N, np);
+    const int nb = na * np;
+    // one buffer: amp, pos, edges, lim, counts, d2, link
+    const size_t o_amp = 0, o_pos = o_amp + sizeof(float) * (size_t)N;
+    const size_t o_ea = (o_pos + sizeof(float) * (size_t)N + 255) & ~(size_t)255;
+    const size_t o_ep = o_ea + sizeof(float) * (size_t)(na + 1);
+    const size_t o_lim = (o_ep + sizeof(float) * (size_t)(np + 1) + 255) & ~(size_t)255;
+    const size_t o_cnt = (o_lim + sizeof(int64_t) * (size_t)(C + 1) + 255) & ~(size_t)255;
+    const size_t o_d2 = (o_cnt + sizeof(int) * (size_t)nb * C + 255) & ~(size_t)255;
+    const size_t o_lnk = (o_d2 + sizeof(int64_t) * (size_t)C * C + 255) & ~(size_t)255;
+    const size_t total = o_lnk + (size_t)C * C;
+    CHK(ensure(ctx, ctx->b_drift, total));
+    char* base = reinterpret_cast<char*>(ctx->b_drift.p);
+    HIPCHK(hipMemcpyAsync(base + o_amp, pr->amp, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_pos, pr->pos, sizeof(float) * N, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_ea, qa.data(), sizeof(float) * qa.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_ep, qp.data(), sizeof(float) * qp.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(base + o_lim, lim.data(), sizeof(int64_t) * lim.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(irc::launch_drift(st, N, C, reinterpret_cast<int64_t*>(base + o_lim), reinterpret_cast<float*>(base + o_amp),
+                             reinterpret_cast<float*>(base + o_pos), reinterpret_cast<float*>(base + o_ea), na,
+                             reinterpret_cast<float*>(base + o_ep), np, pr->n_batch, pr->n_time_max,
+                             reinterpret_cast<int*>(base + o_cnt), reinterpret_cast<int64_t*>(base + o_d2),
+                             reinterpret_cast<uint8_t*>(base + o_lnk)));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    HIPCHK(hipMemcpyAsync(link_out, base + o_lnk, (size_t)C * C, hipMemcpyDeviceToHost, st));
+    if (counts_out)
+        HIPCHK(hipMemcpyAsync(counts_out, base + o_cnt, sizeof(int) * (size_t)nb * C, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    memcpy(lim_out, lim.data(), sizeof(int64_t) * lim.size());
+    *n_drift_out = C;
+    ctx->stats = irc_stats{};
+    ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
     return IRC_OK;
 }
 

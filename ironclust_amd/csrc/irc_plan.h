@@ -32,5 +32,8 @@ hipError_t launch_plan(hipStream_t st, int S, int C, const DevPlan& p, const int
 // device counters of a call (read once after the call's last kernel): *dst = s32 ? *s32 : *s64
 hipError_t launch_stat_copy(hipStream_t st, const int* s32, const int64_t* s64, int64_t* dst);
 hipError_t launch_zero_i64(hipStream_t st, int64_t* p, int n);
+// largest candidate count of a block holding queries (atomicMax into *out, which starts at 0)
+hipError_t launch_max_ncand(hipStream_t st, int S, int C, const int* link_ptr, const int* link_idx, const int* run_len,
+                            int* out);
 
 }  // namespace irc

@@ -175,6 +175,27 @@ __global__ void k_plan_segments(int nb, PlanBuf pb, int* __restrict__ seg_beg, i
 
 }  // namespace
 
+// largest candidate count of any block (thread per block), into *out with atomicMax
+__global__ void k_max_ncand(int S, int C, const int* __restrict__ link_ptr, const int* __restrict__ link_idx,
+                            const int* __restrict__ run_len, int* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= S * C || run_len[2 * b] == 0) return;
+    const int s = b / C, c = b - s * C;
+    int64_t n = 0;
+    for (int li = link_ptr[c]; li < link_ptr[c + 1]; ++li) {
+        const int b2 = s * C + link_idx[li];
+        n += run_len[2 * b2] + run_len[2 * b2 + 1];
+    }
+    atomicMax(out, (int)(n < 0x7fffffff ? n : 0x7fffffff));
+}
+
+hipError_t launch_max_ncand(hipStream_t st, int S, int C, const int* link_ptr, const int* link_idx, const int* run_len,
+                            int* out) {
+    const int nb = S * C;
+    k_max_ncand<<<(nb + 255) / 256, 256, 0, st>>>(S, C, link_ptr, link_idx, run_len, out);
+    return hipGetLastError();
+}
+
 __global__ void k_stat_copy(const int* __restrict__ s32, const int64_t* __restrict__ s64, int64_t* __restrict__ dst) {
     if (threadIdx.x == 0) *dst = s32 ? (int64_t)*s32 : *s64;
 }

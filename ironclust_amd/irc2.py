@@ -10,6 +10,7 @@ are D x n (one column per spike), indices returned are 1-based.
   sort_page_          matlab/irc2.m:3456-3503 (rho_page_, delta_page_, rho_paged_site_,
                       delta_paged_site_ folded into one batched call)
   plan_sort_page_     matlab/irc2.m:3914-3946
+  calc_drift_         matlab/irc2.m:6373-6447 (drift linkage that feeds the path)
   prepare_page_site_  matlab/irc2.m:3869-3910 (site_layout builds the same column grouping
                       for every site at once)
 
@@ -88,6 +89,22 @@ def cuda_knn_(mrFet2, mrFet1, P=None):
     knn = min(knn, B.shape[0])
     K, I = get_ctx(P.get("device", 0)).knn_ab(B, A, knn)
     return K, 1, np.ascontiguousarray(I.T)
+
+
+def calc_drift_(vrAmp_spk, mrPos_spk, P=None):
+    """S_drift = calc_drift_(S0, P) (matlab/irc2.m:6373-6447) on the spikes' amplitudes and
+    positions: returns dict(nTime_drift, mlDrift (C x C logical), viLim_drift (1-based),
+    mrCount_drift).  P keys: nTime_batch, nTime_drift, nQuantile_drift (10), nTime_max_drift (256),
+    viSite2Chan (its length is the number of position bins)."""
+    P = P or {}
+    pos = np.asarray(mrPos_spk, dtype=np.float32)
+    if pos.ndim == 2:
+        pos = pos[:, 1]
+    n_pos = len(P["viSite2Chan"]) if "viSite2Chan" in P else int(P.get("nSites", 1))
+    link, lim, counts = get_ctx(P.get("device", 0)).calc_drift(
+        np.asarray(vrAmp_spk, dtype=np.float32).ravel(), pos, n_pos, int(P.get("nQuantile_drift", 10)),
+        int(P["nTime_drift"]), int(P["nTime_batch"]), int(P.get("nTime_max_drift", 256)), want_counts=True)
+    return dict(nTime_drift=link.shape[0], mlDrift=link, viLim_drift=lim, mrCount_drift=counts)
 
 
 def site_layout(ds, F1=None):

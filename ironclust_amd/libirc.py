@@ -20,6 +20,7 @@ EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
     "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
+    "irc_calc_drift",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -47,6 +48,12 @@ class Batch(ctypes.Structure):
                 ("n_sites", ctypes.c_int32), ("n_drift", ctypes.c_int32), ("n_cols", ctypes.c_int64),
                 ("F", _c_f32p), ("col_spike", _c_i32p), ("site_ptr", _c_i64p),
                 ("site_n_primary", _c_i64p), ("lim_drift", _c_i64p), ("link", _c_u8p)]
+
+
+class DriftProblem(ctypes.Structure):
+    _fields_ = [("n_spikes", ctypes.c_int64), ("amp", _c_f32p), ("pos", _c_f32p), ("n_amp", ctypes.c_int32),
+                ("n_pos", ctypes.c_int32), ("n_drift", ctypes.c_int32), ("n_batch", ctypes.c_int32),
+                ("n_time_max", ctypes.c_int32)]
 
 
 class Outputs(ctypes.Structure):
@@ -117,6 +124,9 @@ def load(path=None):
     lib.irc_delta_drift_dc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _c_i32p,
                                        _c_f32p, _c_f32p, _c_f32p, ctypes.c_float, _c_f32p, _c_u32p]
     lib.irc_delta_drift_dc.restype = ctypes.c_int
+    lib.irc_calc_drift.argtypes = [ctypes.c_void_p, ctypes.POINTER(DriftProblem), _c_i32p, _c_i64p,
+                                   ctypes.POINTER(ctypes.c_uint8), _c_i32p]
+    lib.irc_calc_drift.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -310,6 +320,30 @@ class Context:
         self._check(self.lib.irc_rho_drift_dc(self.h, f1.shape[1], int(n1), f1.shape[0], _p(ms, _c_i32p),
                                               _p(f1, _c_f32p), _p(f2, _c_f32p), float(dc2), _p(rho, _c_f32p)))
         return rho
+
+    def calc_drift(self, amp, pos, n_pos, n_amp, n_drift, n_batch, n_time_max=256, want_counts=False):
+        """Drift linkage (calc_drift_).  Returns (link (C, C) bool, link[r, c] = mlDrift(r, c);
+        lim (C+1,) 1-based; counts ((n_amp*n_pos), C) int32 or None)."""
+        amp = np.ascontiguousarray(amp, dtype=np.float32)
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        pr = DriftProblem()
+        pr.n_spikes = amp.size
+        pr.amp, pr.pos = _p(amp, _c_f32p), _p(pos, _c_f32p)
+        pr.n_amp, pr.n_pos, pr.n_drift, pr.n_batch, pr.n_time_max = n_amp, n_pos, n_drift, n_batch, n_time_max
+        C = int(n_drift)
+        nC = ctypes.c_int32(0)
+        lim = np.zeros(C + 1, np.int64)
+        link = np.zeros(C * C, np.uint8)
+        counts = np.zeros((C, n_amp * n_pos), np.int32) if want_counts else None
+        self._check(self.lib.irc_calc_drift(self.h, ctypes.byref(pr), ctypes.byref(nC), _p(lim, _c_i64p),
+                                            link.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                            _p(counts, _c_i32p)))
+        C = nC.value
+        link = link[:C * C].reshape(C, C, order="F").astype(bool)
+        lim = lim[:C + 1]
+        if counts is not None:
+            counts = np.ascontiguousarray(counts[:C].T) if C > 1 else None
+        return link, lim, counts
 
     def delta_drift_dc(self, n1, miSite12, fet1, fet2, rho12, dc2):
         m = np.asfortranarray(np.asarray(miSite12, dtype=np.int32))
