@@ -748,7 +748,8 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     const int64_t n_cols = b->site_ptr[S];
     if (b->n_cols != 0 && b->n_cols != n_cols) return fail(ctx, IRC_E_ARG, "n_cols does not match site_ptr[S]");
     if (prim[S] != N) return fail(ctx, IRC_E_ARG, "sum of site_n_primary (%lld) != n_spikes (%lld)", (long long)prim[S], (long long)N);
-    if (n_cols >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    // kernels launch up to 6 work-items per column; one dispatch covers at most 2^32
+    if (n_cols > 500000000) return fail(ctx, IRC_E_UNSUPPORTED, "more than 500M columns");
     for (int c = 0; c < C; ++c)
         if (b->lim_drift[c + 1] < b->lim_drift[c]) return fail(ctx, IRC_E_ARG, "lim_drift not monotone");
     if (!links_have_diagonal(C, b->link)) return fail(ctx, IRC_E_ARG, "link(c,c) must be set for every chunk");

@@ -345,13 +345,15 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
 // Planning kernels (batch form).
 
 // Pack D x n (column-major) features into [n][DP] rows, zero padded.
+// grid-stride: a dispatch covers at most 2^32 work-items, fewer than n_cols * DP at 100M columns
 __global__ void k_pack_fp32(int64_t n_cols, int D, int DP, const float* __restrict__ F, float* __restrict__ feat) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = n_cols * DP;
-    if (e >= total) return;
-    const int64_t col = e / DP;
-    const int d = (int)(e - col * DP);
-    feat[e] = d < D ? F[col * D + d] : 0.0f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t col = e / DP;
+        const int d = (int)(e - col * DP);
+        feat[e] = d < D ? F[col * D + d] : 0.0f;
+    }
 }
 
 __device__ __forceinline__ int64_t lower_bound_i32(const int* a, int64_t lo, int64_t hi, int64_t v) {
@@ -425,7 +427,8 @@ namespace irc {
 hipError_t launch_pack_fp32(hipStream_t st, int64_t n_cols, int D, int DP, const float* F, float* feat) {
     const int64_t total = n_cols * DP;
     if (total == 0) return hipSuccess;
-    k_pack_fp32<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(n_cols, D, DP, F, feat);
+    const int64_t blocks = (total + 255) / 256;
+    k_pack_fp32<<<(unsigned)(blocks < (1 << 20) ? blocks : (1 << 20)), 256, 0, st>>>(n_cols, D, DP, F, feat);
     return hipGetLastError();
 }
 

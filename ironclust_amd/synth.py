@@ -65,6 +65,25 @@ def lognormal_counts(n_sites, n_spikes, rng, median=20_000, lo=500, hi=600_000, 
     return n
 
 
+PIECE = 1 << 22
+
+
+def _draw(fn, n, gen, seed, tag, device, **kw):
+    """fn(size, generator=...) over n values in pieces of PIECE: the first piece from the shared
+    generator (so recordings of up to PIECE spikes draw exactly as one call), every further piece
+    from its own generator seeded by (seed, tag, piece) -- one huge draw from a single device
+    generator is avoided, and the pieces are independent streams."""
+    if n <= PIECE:
+        return fn(n, generator=gen, **kw)
+    parts = []
+    for i, a in enumerate(range(0, n, PIECE)):
+        b = min(n, a + PIECE)
+        g = gen if i == 0 else torch.Generator(device=device).manual_seed(
+            (seed * 1_000_003 + tag * 10_007 + i) % (1 << 62))
+        parts.append(fn(b - a, generator=g, **kw))
+    return torch.cat(parts)
+
+
 def make_dataset(n_spikes, n_sites, D, n_drift, n_clusters, k=30, drift_step=0.0, n_links=1, band=0,
                  seed=1, lognormal=False, f2=False, f2_frac=0.5, center_std=4.0, device="cpu",
                  link=None):
@@ -102,7 +121,8 @@ def make_dataset(n_spikes, n_sites, D, n_drift, n_clusters, k=30, drift_step=0.0
         site1 = torch.from_numpy(np.concatenate(site_parts)).to(device)
     else:
         lim = lim_equal(N, C)
-        site1 = torch.randint(1, S + 1, (N,), generator=gen, device=device, dtype=torch.int32)
+        site1 = _draw(lambda n, **kw: torch.randint(1, S + 1, (n,), device=device, dtype=torch.int32, **kw),
+                      N, gen, seed, 1, device)
     chunk = torch.zeros(N, dtype=torch.int64, device=device)
     bounds = torch.from_numpy(lim[1:-1] - 1).to(device)           # 0-based starts of chunks 1..C-1
     if C > 1:
@@ -111,29 +131,30 @@ def make_dataset(n_spikes, n_sites, D, n_drift, n_clusters, k=30, drift_step=0.0
     u = torch.randn((S, D), generator=gen, device=device)
     u = u / u.norm(dim=1, keepdim=True)
     dl = torch.from_numpy(delta.astype(np.float32)).to(device)
-    z = torch.randint(0, n_clusters, (N,), generator=gen, device=device)
+    z = _draw(lambda n, **kw: torch.randint(0, n_clusters, (n,), device=device, **kw), N, gen, seed, 2, device)
     s0 = site1.long() - 1
 
-    def feats(site0, zz):
+    def feats(site0, zz, tag):
         out = torch.empty((N, D), dtype=torch.float32, device=device)
-        step = 1 << 22
-        for a in range(0, N, step):
-            b = min(N, a + step)
-            x = torch.randn((b - a, D), generator=gen, device=device)
+        for i, a in enumerate(range(0, N, PIECE)):
+            b = min(N, a + PIECE)
+            g = gen if i == 0 else torch.Generator(device=device).manual_seed(
+                (seed * 1_000_003 + tag * 10_007 + i) % (1 << 62))
+            x = torch.randn((b - a, D), generator=g, device=device)
             x += centers[site0[a:b], zz[a:b]]
             x += dl[chunk[a:b]][:, None] * u[site0[a:b]]
             out[a:b] = x
         return out
 
-    F1 = feats(s0, z)
+    F1 = feats(s0, z, 3)
     F2 = site2 = None
     if f2 and S > 1:
-        has2 = torch.rand((N,), generator=gen, device=device) < f2_frac
-        off = torch.randint(1, S, (N,), generator=gen, device=device)
+        has2 = _draw(lambda n, **kw: torch.rand((n,), device=device, **kw), N, gen, seed, 4, device) < f2_frac
+        off = _draw(lambda n, **kw: torch.randint(1, S, (n,), device=device, **kw), N, gen, seed, 5, device)
         s2 = (s0 + off) % S
         site2 = torch.where(has2, (s2 + 1).int(), torch.zeros_like(site1))
         z2 = (z + 1) % n_clusters
-        F2 = feats(s2, z2)
+        F2 = feats(s2, z2, 6)
         F2[~has2] = 0.0
     return dict(n_spikes=N, n_sites=S, D=D, n_drift=C, k=k, F1=F1, F2=F2, site1=site1, site2=site2,
                 lim=lim, link=link, delta=delta)
