@@ -43,7 +43,7 @@ struct irc_ctx {
     int large_ncand = 8192;
     // device buffers, grown on demand and kept across calls
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
-    DevBuf b_small, b_flags, b_knncol, b_knnt, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
+    DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
     DevBuf b_scr[8];
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
@@ -343,7 +343,7 @@ struct DevWork {
 // from `ranges` (host planning, per-site entry points) or is built on the device (`dw`, batch).
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
                    int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
-                   double* knn_t, double* dk2) {
+                   double* dk2) {
     hipStream_t st = ctx->stream;
     int64_t pairs = 0;
     if (ranges)
@@ -361,7 +361,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     if (!screen) {
         ctx->stats.precision_used = IRC_PREC_EXACT;
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
-        HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2, nullptr));
+        HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, dk2, nullptr));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
         ctx->stats.n_screen_fallback = nq;
         return IRC_OK;
@@ -486,10 +486,10 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
     int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
-    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, knn_t, dk2));
+    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2));
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
-    HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, knn_t, dk2));
+    HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 1));
     if (!ranges) HIPCHK(irc::launch_stat_copy(st, nullptr, pb.n_pairs, dstat + 2));
     return IRC_OK;
@@ -609,7 +609,7 @@ void irc_ctx_destroy(irc_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf* all[] = {&ctx->b_F, &ctx->b_colspk, &ctx->b_feat, &ctx->b_key, &ctx->b_ret, &ctx->b_runb,
                      &ctx->b_runl, &ctx->b_lptr, &ctx->b_lidx, &ctx->b_qcol, &ctx->b_qblk, &ctx->b_qout,
-                     &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_knnt, &ctx->b_dk2, &ctx->b_rhoslot,
+                     &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_dk2, &ctx->b_rhoslot,
                      &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
                      &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
                      &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan, &ctx->b_dstat,
@@ -659,7 +659,6 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     CHK(upload_plan(ctx, h, p));
     const int k = p.k;
     CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
-    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)nq * k));
     CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
     CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
     CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
@@ -677,7 +676,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols,
-                &ranges, nullptr, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+                &ranges, nullptr, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
@@ -798,7 +797,6 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     CHK(ensure(ctx, ctx->b_qout, sizeof(int) * (size_t)N));
     CHK(ensure(ctx, ctx->b_flags, sizeof(int) * 4));
     CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)N * k));
-    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)N * k));
     CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
     CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
     CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
@@ -878,14 +876,13 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
         CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
-                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
         HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
                                       o_knn, nullptr));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
         HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
-        HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt),
-                                          ptr<float>(ctx->b_rhocol), o_delta, o_nn, ptr<int>(ctx->b_unres),
-                                          d_nunres));
+        HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol), o_delta,
+                                          o_nn, ptr<int>(ctx->b_unres), d_nunres));
         HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
                                        ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
         HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->b_dstat) + 3));
@@ -982,7 +979,6 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     CHK(upload_plan(ctx, h, p));
     const int nq = (int)nA;
     CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
-    CHK(ensure(ctx, ctx->b_knnt, sizeof(double) * (size_t)nq * k));
     CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
     CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
     CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
@@ -991,7 +987,7 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     ctx->stats.n_queries = nq;
     const std::vector<QRange> no_ranges;
     CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, &no_ranges, nullptr,
-                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_knnt), ptr<double>(ctx->b_dk2)));
+                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
                                   ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));

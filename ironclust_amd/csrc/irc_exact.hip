@@ -43,12 +43,12 @@ __device__ __forceinline__ bool less_tc(const DevPlan& p, double t, int key, dou
 // query row held in registers, and keeps its own sorted list of up to k entries in LDS; k
 // rounds of a wave-wide argmin then merge the 64 lists.  The next batch is in flight while a
 // batch is scored.
-// Writes knn_col / knn_t (slot-major: entry j of query q at j * p.nq + q) and dk2 per query.
+// Writes knn_col (slot-major: entry j of query q at j * p.nq + q) and dk2 per query.
 template <int NV>
 __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restrict__ qlist, int nq_host,
                                                   const int* __restrict__ nq_dev,
-                                                  int* __restrict__ knn_col, double* __restrict__ knn_t,
-                                                  double* __restrict__ dk2, int* __restrict__ n_cand) {
+                                                  int* __restrict__ knn_col, double* __restrict__ dk2,
+                                                  int* __restrict__ n_cand) {
     extern __shared__ double smem[];
     float4* stage = reinterpret_cast<float4*>(smem);                  // [64][NV] rows of the batch
     double* lt = smem + 64 * NV * 2;                                   // [k][64] distances
@@ -164,7 +164,6 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
             if (wl == lane) ++head;
             if (lane == 0) {
                 knn_col[(int64_t)j * p.nq + q] = col;
-                knn_t[(int64_t)j * p.nq + q] = t;
             }
             last_t = t;
             last_c = col;
@@ -172,7 +171,6 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
         if (lane == 0) {
             for (int j = kk; j < k; ++j) {
                 knn_col[(int64_t)j * p.nq + q] = last_c;
-                knn_t[(int64_t)j * p.nq + q] = last_t;
             }
             dk2[q] = last_t;
             if (n_cand) n_cand[q] = total;
@@ -229,11 +227,12 @@ __global__ void k_rho_cols(int64_t n_cols, const int* __restrict__ col_key, cons
 // ---------------------------------------------------------------------------
 // Delta from the kNN list: the first kNN entry (in (d2,id) order) whose rho exceeds
 // the query's is the nearest denser spike, because kNN(a) is a prefix of C(a) in key
-// order.  Queries without such an entry are appended to `unres`.
+// order; its canonical distance is recomputed here (one row pair per query).  Queries
+// without such an entry are appended to `unres`.
 __global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_col,
-                                 const double* __restrict__ knn_t, const float* __restrict__ rho_col,
-                                 float* __restrict__ delta_slot, int64_t* __restrict__ nneigh_slot,
-                                 int* __restrict__ unres, int* __restrict__ n_unres) {
+                                 const float* __restrict__ rho_col, float* __restrict__ delta_slot,
+                                 int64_t* __restrict__ nneigh_slot, int* __restrict__ unres,
+                                 int* __restrict__ n_unres) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const int qc = p.q_col[q];
@@ -243,7 +242,18 @@ __global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_
         const int col = knn_col[(int64_t)j * p.nq + q];
         if (rho_col[col] > ra) {
             const int slot = p.q_out[q];
-            const float dl = (float)(sqrt(knn_t[(int64_t)j * p.nq + q]) * (double)ra);
+            const float4* a4 = reinterpret_cast<const float4*>(p.feat + (int64_t)qc * p.DP);
+            const float4* b4 = reinterpret_cast<const float4*>(p.feat + (int64_t)col * p.DP);
+            double t = 0.0;
+            for (int i = 0; i < p.DP / 4; ++i) {
+                const float4 x = a4[i], y = b4[i];
+                double d;
+                d = (double)x.x - (double)y.x; t = fma(d, d, t);
+                d = (double)x.y - (double)y.y; t = fma(d, d, t);
+                d = (double)x.z - (double)y.z; t = fma(d, d, t);
+                d = (double)x.w - (double)y.w; t = fma(d, d, t);
+            }
+            const float dl = (float)(sqrt(t) * (double)ra);
             if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
                 delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
                 nneigh_slot[slot] = p.col_ret[qc];
@@ -455,12 +465,12 @@ size_t knn_exact_lds_bytes(int k, int DP) { return (size_t)64 * DP * 4 + (size_t
 
 namespace {
 hipError_t knn_exact_go(hipStream_t st, const DevPlan& p, const int* qlist, int nq, const int* nq_dev, int grid,
-                        int* knn_col, double* knn_t, double* dk2, int* n_cand) {
+                        int* knn_col, double* dk2, int* n_cand) {
     const int g = grid < nq ? grid : nq;
     const size_t lds = knn_exact_lds_bytes(p.k, p.DP);
 #define IRC_EX(NV)                                                                                           \
     case NV:                                                                                                 \
-        k_knn_exact<NV><<<g, 64, lds, st>>>(p, qlist, nq, nq_dev, knn_col, knn_t, dk2, n_cand);              \
+        k_knn_exact<NV><<<g, 64, lds, st>>>(p, qlist, nq, nq_dev, knn_col, dk2, n_cand);                     \
         break;
     switch (p.DP / 4) {
         IRC_EX(1) IRC_EX(2) IRC_EX(3) IRC_EX(4) IRC_EX(5) IRC_EX(6) IRC_EX(7) IRC_EX(8)
@@ -473,15 +483,15 @@ hipError_t knn_exact_go(hipStream_t st, const DevPlan& p, const int* qlist, int 
 }  // namespace
 
 hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, int grid,
-                            int* knn_col, double* knn_t, double* dk2, int* n_cand) {
+                            int* knn_col, double* dk2, int* n_cand) {
     if (nq <= 0) return hipSuccess;
-    return knn_exact_go(st, p, qlist, nq, nullptr, grid, knn_col, knn_t, dk2, n_cand);
+    return knn_exact_go(st, p, qlist, nq, nullptr, grid, knn_col, dk2, n_cand);
 }
 
 hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,
-                                int grid, int* knn_col, double* knn_t, double* dk2) {
+                                int grid, int* knn_col, double* dk2) {
     if (nq_max <= 0) return hipSuccess;
-    return knn_exact_go(st, p, qlist, nq_max, nq_dev, grid, knn_col, knn_t, dk2, nullptr);
+    return knn_exact_go(st, p, qlist, nq_max, nq_dev, grid, knn_col, dk2, nullptr);
 }
 
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
@@ -498,11 +508,11 @@ hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, c
     return hipGetLastError();
 }
 
-hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, const double* knn_t,
+hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col,
                                  const float* rho_col, float* delta_slot, int64_t* nneigh_slot, int* unres,
                                  int* n_unres) {
     if (nq <= 0) return hipSuccess;
-    k_delta_from_knn<<<(nq + 255) / 256, 256, 0, st>>>(p, nq, knn_col, knn_t, rho_col, delta_slot, nneigh_slot,
+    k_delta_from_knn<<<(nq + 255) / 256, 256, 0, st>>>(p, nq, knn_col, rho_col, delta_slot, nneigh_slot,
                                                        unres, n_unres);
     return hipGetLastError();
 }

@@ -13,15 +13,15 @@ hipError_t launch_batch_queries(hipStream_t st, int64_t N, int S, int C, const i
                                 int* flags);
 size_t knn_exact_lds_bytes(int k, int DP);
 hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, int grid,
-                            int* knn_col, double* knn_t, double* dk2, int* n_cand);
+                            int* knn_col, double* dk2, int* n_cand);
 hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,
-                                int grid, int* knn_col, double* knn_t, double* dk2);
+                                int grid, int* knn_col, double* dk2);
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
                              const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32);
 hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, const float* rho_slot,
                            float* rho_col);
 hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col,
-                                 const double* knn_t, const float* rho_col, float* delta_slot,
+                                 const float* rho_col, float* delta_slot,
                                  int64_t* nneigh_slot, int* unres, int* n_unres);
 hipError_t launch_delta_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, const int* nq_dev,
                               int grid, int mode, const float* rho_col, float* delta_slot, int64_t* nneigh64,

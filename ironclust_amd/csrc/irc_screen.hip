@@ -523,11 +523,12 @@ template <int NV>
 __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int* __restrict__ cand_n,
                                               const int* __restrict__ cand_col, const float* __restrict__ cand_d,
                                               const double* __restrict__ cand_eps, int* __restrict__ knn_col,
-                                              double* __restrict__ knn_t, double* __restrict__ dk2,
+                                              double* __restrict__ dk2,
                                               int* __restrict__ fallback, int* __restrict__ n_fallback) {
     __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
     __shared__ int skey[64];            // their spike ids
     __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
+    extern __shared__ int obuf[];       // [k][64] emitted columns when k <= 32 (else written directly)
     const int l = threadIdx.x;
     // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
     // of the query groups, so the candidate rows its groups share stay in its L2
@@ -625,8 +626,8 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
                 }
                 ++wn;
                 while (wn > 0 && out < kk && wt[0] < lb) {
-                    knn_col[(int64_t)out * n_q + q] = wc[0];
-                    knn_t[(int64_t)out * n_q + q] = wt[0];
+                    if (k <= 32) obuf[out * 64 + l] = wc[0];
+                    else knn_col[(int64_t)out * n_q + q] = wc[0];
                     last_t = wt[0];
                     last_c = wc[0];
                     ++out;
@@ -641,16 +642,21 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
         colj = coln;
         coln = coln2;
     }
-    if (!qv || rn < 0) return;
-    if (ok && out == kk) {
+    const bool done = qv && rn >= 0 && ok && out == kk;
+    if (done) {
         for (int j = kk; j < k; ++j) {
-            knn_col[(int64_t)j * n_q + q] = last_c;
-            knn_t[(int64_t)j * n_q + q] = last_t;
+            if (k <= 32) obuf[j * 64 + l] = last_c;
+            else knn_col[(int64_t)j * n_q + q] = last_c;
         }
         dk2[q] = last_t;
-    } else {
+    } else if (qv && rn >= 0) {
         const int f = atomicAdd(n_fallback, 1);
         fallback[f] = (int)q;
+    }
+    // the buffered lists leave slot by slot: 64 consecutive queries per store
+    if (k <= 32) {
+        for (int j = 0; j < k; ++j)
+            if (done) knn_col[(int64_t)j * n_q + q] = obuf[j * 64 + l];
     }
 }
 
@@ -773,12 +779,13 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArg
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* knn_t, double* dk2) {
+hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2) {
     if (a.n_q <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
+    const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
-        k_rerank<NV><<<grid, 64, 0, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, knn_t, \
+        k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, \
                                           dk2, a.fallback, a.n_fallback);                                  \
         break;
     switch (a.p.DP / 4) {
