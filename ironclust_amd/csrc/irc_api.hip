@@ -443,10 +443,11 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     }
     const int KP = irc::screen_kp(p.D);
     const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
-    CHK(ensure(ctx, ctx->b_scr[0], eb * (size_t)n_cols * KP));
-    CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
-    CHK(ensure(ctx, ctx->b_scr[6], eb * (size_t)n_cols * KP));
-    CHK(ensure(ctx, ctx->b_scr[7], sp == irc::SCREEN_FP32 ? 256 : eb * (size_t)n_cols * KP));
+    const size_t pcols = (size_t)irc::screen_cols_padded(n_cols);   // tiled rows, whole 32-column groups
+    CHK(ensure(ctx, ctx->b_scr[0], eb * pcols * KP));
+    CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
+    CHK(ensure(ctx, ctx->b_scr[6], eb * pcols * KP));
+    CHK(ensure(ctx, ctx->b_scr[7], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
     CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
     // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
     const int slots = (kprime > 32 && any_large) ? 64 : kprime;

@@ -16,12 +16,21 @@ struct ScreenItem {
 #define IRC_SCREEN_MAXCAND 65536  // candidate positions fit 16 bits of the packed key
 #define IRC_SCREEN_MAXRUNS 128    // runs per block: 2 per linked chunk
 
+// Packed operand rows are stored in 16-byte pieces (8 bf16 or 4 fp32 values), tiled by groups
+// of 32 columns: [col / 32][piece][col % 32], so one wavefront load of piece p over 32
+// consecutive columns reads 512 contiguous bytes.  np = pieces per row; the result is the
+// piece's index in 16-byte units.  Buffers hold screen_cols_padded(n_cols) rows.
+__host__ __device__ inline int64_t screen_piece(int64_t col, int piece, int np) {
+    return (((col >> 5) * np + piece) << 5) | (col & 31);
+}
+inline int64_t screen_cols_padded(int64_t n_cols) { return (n_cols + 31) & ~(int64_t)31; }
+
 struct ScreenArgs {
     DevPlan p;
-    const void* hi;          // [n_cols][KP] bf16 bits (bf16 form) or fp32 rows (fp32 form)
-    const void* lo;          // [n_cols][KP] bf16 low parts (bf16 form)
-    const void* qhi;         // [n_cols][KP] query-form rows [-2x, |x|^2, 1, 0...] (hi parts or fp32)
-    const void* qlo;         // [n_cols][KP] query-form low parts (bf16 form)
+    const void* hi;          // tiled rows: bf16 bits (bf16 form) or fp32 values (fp32 form)
+    const void* lo;          // tiled rows: bf16 low parts (bf16 form)
+    const void* qhi;         // tiled query-form rows [-2x, |x|^2, 1, 0...] (hi parts or fp32)
+    const void* qlo;         // tiled query-form low parts (bf16 form)
     const float* norm;       // [n_cols] fp32 |x|^2
     const float* run_nmax;   // [n_blocks * 2] largest |x|^2 of each (block, part) run
     const ScreenItem* items;

@@ -86,16 +86,18 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
         cv[t] = d < D ? xv : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
         qv[t] = d < D ? xv : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
     }
-    const int64_t off = col * KP + 8 * j;    if (prec == SCREEN_FP32) {
-        float4* h4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(hi) + off);
-        float4* q4 = reinterpret_cast<float4*>(reinterpret_cast<float*>(qhi) + off);
-        h4[0] = make_float4(cv[0], cv[1], cv[2], cv[3]);
-        h4[1] = make_float4(cv[4], cv[5], cv[6], cv[7]);
+    if (prec == SCREEN_FP32) {
+        // fp32 rows are KP / 4 pieces: this thread's 8 values are pieces 2 j and 2 j + 1
+        float4* h4 = reinterpret_cast<float4*>(hi);
+        float4* q4 = reinterpret_cast<float4*>(qhi);
+        const int np = KP / 4;
+        h4[screen_piece(col, 2 * j, np)] = make_float4(cv[0], cv[1], cv[2], cv[3]);
+        h4[screen_piece(col, 2 * j + 1, np)] = make_float4(cv[4], cv[5], cv[6], cv[7]);
         float s[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) s[t] = 8 * j + t < D ? -2.0f * qv[t] : qv[t];
-        q4[0] = make_float4(s[0], s[1], s[2], s[3]);
-        q4[1] = make_float4(s[4], s[5], s[6], s[7]);
+        q4[screen_piece(col, 2 * j, np)] = make_float4(s[0], s[1], s[2], s[3]);
+        q4[screen_piece(col, 2 * j + 1, np)] = make_float4(s[4], s[5], s[6], s[7]);
     } else {
         uint32_t ph[4], pl[4], qh[4], ql[4];
 #pragma unroll
@@ -111,10 +113,12 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
             qh[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(a0)) | ((uint32_t)f2bf_rne(w1 * bf2f(a1)) << 16);
             ql[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(b0)) | ((uint32_t)f2bf_rne(w1 * bf2f(b1)) << 16);
         }
-        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(hi) + off) = make_uint4(ph[0], ph[1], ph[2], ph[3]);
-        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(lo) + off) = make_uint4(pl[0], pl[1], pl[2], pl[3]);
-        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(qhi) + off) = make_uint4(qh[0], qh[1], qh[2], qh[3]);
-        *reinterpret_cast<uint4*>(reinterpret_cast<uint16_t*>(qlo) + off) = make_uint4(ql[0], ql[1], ql[2], ql[3]);
+        // bf16 rows are KP / 8 pieces: this thread's 8 values are piece j
+        const int64_t o = screen_piece(col, j, KP / 8);
+        reinterpret_cast<uint4*>(hi)[o] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
+        reinterpret_cast<uint4*>(lo)[o] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+        reinterpret_cast<uint4*>(qhi)[o] = make_uint4(qh[0], qh[1], qh[2], qh[3]);
+        reinterpret_cast<uint4*>(qlo)[o] = make_uint4(ql[0], ql[1], ql[2], ql[3]);
     }
 }
 
@@ -138,6 +142,25 @@ __device__ __forceinline__ void bitonic_sort(int32_t* v) {
             for (int i = 0; i < N; ++i) {
                 const int l = i ^ j;
                 if (l > i) cex<N>(v[i], v[l], (i & k) == 0);
+            }
+        }
+    }
+}
+
+// Batcher's odd-even merge sort (ascending): 191 compare-exchanges for 32 keys against the
+// bitonic network's 240
+template <int N>
+__device__ __forceinline__ void oddeven_sort(int32_t* v) {
+#pragma unroll
+    for (int p = 1; p < N; p <<= 1) {
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+            for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+                for (int i = 0; i < k; ++i) {
+                    if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) cex<N>(v[i + j], v[i + j + k], true);
+                }
             }
         }
     }
@@ -187,66 +210,82 @@ constexpr int32_t KEY_NONE = (int32_t)0x80000000;   // threshold of lanes withou
 
 constexpr int WIN = 16;  // re-rank window: longest run of mutually ambiguous neighbours
 
-template <int PREC, int KB, int LS, bool HIST>
+template <int PREC, int KB, int KPB, bool HIST>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
                                                    const int* __restrict__ seg_beg) {
     constexpr int KP = 16 * KB;
+    constexpr int LS = KPB <= 32 ? 32 : 64;   // list slots in registers
     __shared__ uint32_t sbuf[65 * 64];   // rows 0..63: append buffer / histogram / list; row 64: sink
     __shared__ int s_rb[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_ro[IRC_SCREEN_MAXRUNS + 1];
-    __shared__ int s_nr, s_item;
-    __shared__ float s_nmax;
+    __shared__ int s_item;
     const DevPlan& p = a.p;
     const int lane = threadIdx.x;
     const int h = lane >> 5, m = lane & 31;
-    const int kprime_base = a.kprime;
     // workgroups are dispatched round-robin over the 8 XCDs: blockIdx & 7 names the XCD
     const int my_seg = blockIdx.x & 7;
-    for (;;) {
-        // ---- dequeue: own XCD's segment first (its blocks share candidate chunks in
-        // that XCD's L2), then the other segments; every wave ends when all are empty
-        __syncthreads();
-        if (lane == 0) {
-            int got = -1;
-            for (int t = 0; t < 8 && got < 0; ++t) {
-                const int sg = (my_seg + t) & 7;
-                if (seg_ctr[sg] >= seg_beg[sg + 1] - seg_beg[sg]) continue;
-                const int idx = atomicAdd(&seg_ctr[sg], 1);
-                if (seg_beg[sg] + idx < seg_beg[sg + 1]) got = seg_beg[sg] + idx;
-            }
-            s_item = got;
+    // claim of an item (lane 0): own XCD's segment first (its blocks share candidate chunks in
+    // that XCD's L2), then the other segments; -1 once all are empty, which ends every wave
+    auto claim = [&]() -> int {
+        int got = -1;
+        for (int t = 0; t < 8 && got < 0; ++t) {
+            const int sg = (my_seg + t) & 7;
+            if (seg_ctr[sg] >= seg_beg[sg + 1] - seg_beg[sg]) continue;
+            const int idx = atomicAdd(&seg_ctr[sg], 1);
+            if (seg_beg[sg] + idx < seg_beg[sg + 1]) got = seg_beg[sg] + idx;
         }
+        return got;
+    };
+    // the next item is claimed while the current one is finalized
+    int next = 0;
+    if (lane == 0) next = claim();
+    for (;;) {
+        __syncthreads();
+        if (lane == 0) s_item = next;
         __syncthreads();
         const int it = s_item;
         if (it < 0) break;
         const ScreenItem item = a.items[it];
         const int s = item.blk / p.C, c = item.blk - s * p.C;
-        if (lane == 0) {
-            int nr = 0, off = 0;
-            float nm = 0.0f;
-            for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
-                const int b2 = s * p.C + p.link_idx[li];
-                for (int part = 0; part < 2; ++part) {
-                    const int len = p.run_len[2 * b2 + part];
-                    if (len > 0) {
-                        s_rb[nr] = p.run_beg[2 * b2 + part];
-                        s_ro[nr] = off;
-                        off += len;
-                        nm = fmaxf(nm, a.run_nmax[2 * b2 + part]);
-                        ++nr;
-                    }
+        // the block's candidate runs (linked chunks in link order, primary part then
+        // secondary), one lane per (link, part): stream offsets by a wave prefix sum
+        const int lp0 = p.link_ptr[c], ne = 2 * (p.link_ptr[c + 1] - lp0);
+        int nr = 0, off = 0;
+        float nmax_w = 0.0f;   // largest |c|^2 of the block's candidates
+        for (int e0 = 0; e0 < ne; e0 += 64) {
+            const int e = e0 + lane;
+            int len = 0, beg = 0;
+            float mx = 0.0f;
+            if (e < ne) {
+                const int idx = 2 * (s * p.C + p.link_idx[lp0 + (e >> 1)]) + (e & 1);
+                len = p.run_len[idx];
+                if (len > 0) {
+                    beg = p.run_beg[idx];
+                    mx = a.run_nmax[idx];
                 }
             }
-            s_ro[nr] = off;
-            s_nr = nr;
-            s_nmax = nm;
+            const uint64_t bal = __ballot(len > 0);
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            int incl = len;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += t;
+            }
+            if (len > 0) {
+                s_rb[nr + rank] = beg;
+                s_ro[nr + rank] = off + incl - len;
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) mx = fmaxf(mx, __shfl_xor(mx, d, 64));
+            nmax_w = fmaxf(nmax_w, mx);
+            off += __shfl(incl, 63, 64);
+            nr += __popcll(bal);
         }
+        if (lane == 0) s_ro[nr] = off;
         __syncthreads();
-        const int nr = s_nr;
         const int ncand = item.ncand;
-        // long candidate streams keep a deeper list: their keys carry more position bits (coarser
-        // distances) and their neighbour distances are closer together
-        const int kprime = (LS == 64 && ncand > a.large_ncand) ? 64 : kprime_base;
 
         // ---- query operands (B) for the two 32-query column tiles (query-form rows)
         bf16x8 bh[2][KB], bl[2][KB];
@@ -257,21 +296,20 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const bool qv = qj < item.nq;
             const int qc = p.q_col[item.q0 + (qv ? qj : 0)];
             if constexpr (PREC == SCREEN_BF16) {
-                const float4* rh = reinterpret_cast<const float4*>(reinterpret_cast<const uint16_t*>(a.qhi) +
-                                                                   (int64_t)qc * KP + 8 * h);
-                const float4* rl = reinterpret_cast<const float4*>(reinterpret_cast<const uint16_t*>(a.qlo) +
-                                                                   (int64_t)qc * KP + 8 * h);
+                const float4* rh = reinterpret_cast<const float4*>(a.qhi);
+                const float4* rl = reinterpret_cast<const float4*>(a.qlo);
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
                     const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                    bh[T][kb] = __builtin_bit_cast(bf16x8, qv ? rh[2 * kb] : z);
-                    bl[T][kb] = __builtin_bit_cast(bf16x8, qv ? rl[2 * kb] : z);
+                    const int64_t o = screen_piece(qc, 2 * kb + h, 2 * KB);
+                    bh[T][kb] = __builtin_bit_cast(bf16x8, qv ? rh[o] : z);
+                    bl[T][kb] = __builtin_bit_cast(bf16x8, qv ? rl[o] : z);
                 }
             } else {
-                const float4* r = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(a.qhi) +
-                                                                  (int64_t)qc * KP + 4 * h);
+                const float4* r = reinterpret_cast<const float4*>(a.qhi);
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) bq[T][g] = qv ? r[2 * g] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int g = 0; g < 2 * KB; ++g)
+                    bq[T][g] = qv ? r[screen_piece(qc, 2 * g + h, 4 * KB)] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
         // after the lane swap, lane l owns query l of the item
@@ -279,7 +317,14 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         const int q_mine = item.q0 + (qmine ? lane : 0);
         const int qc_mine = p.q_col[q_mine];
         const float nq_mine = a.norm[qc_mine];
+        // retire the item's own loads here: the stream loop then waits on operand loads only, so
+        // the compiler can keep the second operand buffer's loads in flight (vmcnt(6), not 0)
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt and lgkmcnt unconstrained
 
+        // the item's list depth K' is a compile-time constant: slots past it hold KEY_EMPTY
+        // between merges, so the merge networks lose their dead half
+        auto body = [&](auto kp_c) {
+        constexpr int kprime = decltype(kp_c)::value;
         int32_t L[LS];
 #pragma unroll
         for (int i = 0; i < LS; ++i) L[i] = KEY_EMPTY;
@@ -287,33 +332,47 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         int cnt = 0;
         const int ib = ceil_log2(ncand > 2 ? ncand : 2);
         const uint32_t pmask = (1u << ib) - 1u, hmask = ~pmask;
+        // the position mask in a vector register, so the key is one v_bfi_b32 with the scalar
+        // position as its only scalar operand
+        uint32_t pmask_v;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(pmask_v) : "s"(pmask));
         int run = 0;
 
         // the lane's K'-th kept key: the append threshold (L may hold up to LS >= K' keys; those
         // past K' are larger than the threshold and never change the result)
-        auto kth = [&]() -> int32_t {
-            if constexpr (LS == 32) return L[31];
-            else return kprime == 40 ? L[39] : (kprime == 48 ? L[47] : (kprime == 32 ? L[31] : L[63]));
-        };
+        auto kth = [&]() -> int32_t { return L[kprime - 1]; };
         int32_t tau_init = tau;
         // merge the oldest 32 buffered keys into L, move the rest (at most 32) down
+        // Buffer rows of a lane past its count hold KEY_EMPTY or keys that failed the threshold
+        // when written (larger than L[K'-1] then and ever after, so a merge never keeps them);
+        // rows already merged are cleared.  Without the pre-pass the merge therefore reads the
+        // rows unmasked; with it the threshold can sit below L[K'-1] and the rows are masked.
         auto compact_half = [&]() {
             int32_t B[32];
 #pragma unroll
             for (int i = 0; i < 32; ++i) {
                 const int32_t v = (int32_t)sbuf[i * 64 + lane];
-                B[i] = i < cnt ? v : KEY_EMPTY;
+                B[i] = (!HIST || i < cnt) ? v : KEY_EMPTY;
             }
-            bitonic_sort<32>(B);
+            oddeven_sort<32>(B);
 #pragma unroll
             for (int i = 0; i < 32; ++i) {
                 const int32_t v = B[31 - i];
                 L[LS - 32 + i] = L[LS - 32 + i] < v ? L[LS - 32 + i] : v;
             }
             bitonic_clean<LS>(L);
+#pragma unroll
+            for (int i = kprime; i < LS; ++i) L[i] = KEY_EMPTY;
             if (__any(cnt > 32)) {
 #pragma unroll
                 for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = sbuf[(32 + i) * 64 + lane];
+                if constexpr (!HIST) {
+#pragma unroll
+                    for (int i = 32; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
+                }
+            } else if constexpr (!HIST) {
+#pragma unroll
+                for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
             }
             cnt = cnt > 32 ? cnt - 32 : 0;
             const int32_t t = kth();
@@ -322,7 +381,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 
         // operand rows of the lane's candidate at stream position pos (bf16: hi then lo halves)
         constexpr int NA = PREC == SCREEN_BF16 ? 2 * KB : 2 * KB;
-        float4 An[NA];
+        float4 An0[NA], An1[NA];   // operands of the next two tiles
         // tile [pos0, pos0 + 32): `run` is the run holding pos0, with its start column, stream
         // offset and the next run's offset kept wave-uniform (scalar registers); runs that start
         // inside the tile are walked with uniform branches, each lane keeping its column
@@ -333,7 +392,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             run_ro = __builtin_amdgcn_readfirstlane(s_ro[rr]);
             run_next = rr + 1 < nr ? __builtin_amdgcn_readfirstlane(s_ro[rr + 1]) : 0x7fffffff;
         };
-        auto issue = [&](int pos0) {
+        auto issue = [&](int pos0, float4 (&An)[NA]) {
             while (pos0 >= run_next) run_enter(run + 1);
             const int pos = pos0 + m;
             int col = run_rb + (pos - run_ro);
@@ -345,30 +404,33 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 }
             }
             if (pos >= ncand) col = run_rb;
+            // tiled layout: a load instruction reads piece p of 32 consecutive columns from
+            // 512 contiguous bytes (two such runs when the tile straddles a 32-column group)
             if constexpr (PREC == SCREEN_BF16) {
-                const float4* rh = reinterpret_cast<const float4*>(
-                    reinterpret_cast<const uint16_t*>(a.hi) + (int64_t)col * KP + 8 * h);
-                const float4* rl = reinterpret_cast<const float4*>(
-                    reinterpret_cast<const uint16_t*>(a.lo) + (int64_t)col * KP + 8 * h);
+                const float4* rh = reinterpret_cast<const float4*>(a.hi);
+                const float4* rl = reinterpret_cast<const float4*>(a.lo);
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    An[2 * kb] = rh[2 * kb];
-                    An[2 * kb + 1] = rl[2 * kb];
+                    const int64_t o = screen_piece(col, 2 * kb + h, 2 * KB);
+                    An[2 * kb] = rh[o];
+                    An[2 * kb + 1] = rl[o];
                 }
             } else {
-                const float4* r = reinterpret_cast<const float4*>(
-                    reinterpret_cast<const float*>(a.hi) + (int64_t)col * KP + 4 * h);
+                const float4* r = reinterpret_cast<const float4*>(a.hi);
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) An[g] = r[2 * g];
+                for (int g = 0; g < 2 * KB; ++g) An[g] = r[screen_piece(col, 2 * g + h, 4 * KB)];
             }
         };
         // one pass over the candidate stream: MFMA distance tiles, then consume(v, cpos, valid)
         // for the 32 distances of the lane's query, then after_tile()
         auto stream = [&](auto&& consume, auto&& after_tile) {
             run_enter(0);
-            issue(0);
-            for (int base = 0; base < ncand; base += 32) {
-                const float4* Ac = An;
+            issue(0, An0);
+            issue(32, An1);
+            // tiles alternate between the two operand buffers; every tile issues its MFMAs and
+            // the loads two tiles ahead unconditionally (positions past the stream read a valid
+            // row), so the count of loads in flight is the same on every path
+            auto tile = [&](int base, float4 (&Ac)[NA]) {
                 f32x16 acc0 = {}, acc1 = {};
                 if constexpr (PREC == SCREEN_BF16) {
 #pragma unroll
@@ -396,25 +458,36 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                         acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
                     }
                 }
-                // the operand registers are free once the MFMAs have read them: the next tile's
-                // loads are in flight while this tile's distances are consumed
-                if (base + 32 < ncand) issue(base + 32);
+                // the operand registers are free once the MFMAs have read them: the loads of the
+                // tile after next are in flight while this tile and the next are consumed
+                issue(base + 64, Ac);
+                if (base >= ncand) return;
                 // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
+                uint32_t u[32];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]),
+                                                                    __float_as_uint(acc1[r]), false, false);
+                    u[2 * r] = sw[0];
+                    u[2 * r + 1] = sw[1];
+                }
                 auto tile_out = [&](auto full) {
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]),
-                                                                        __float_as_uint(acc1[r]), false, false);
 #pragma unroll
                         for (int hh = 0; hh < 2; ++hh) {
                             const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
-                            consume((int)sw[hh], cpos, decltype(full)::value || cpos < ncand);
+                            consume((int)u[2 * r + hh], cpos, decltype(full)::value || cpos < ncand);
                         }
                     }
                 };
                 if (base + 32 <= ncand) tile_out(std::true_type{});
                 else tile_out(std::false_type{});
                 after_tile();
+            };
+            for (int base = 0; base < ncand; base += 64) {
+                tile(base, An0);
+                tile(base + 32, An1);
             }
         };
 
@@ -452,23 +525,54 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 
         // ---- pass 2: keys (the screened distance's bits, sign kept, above the candidate's stream
         // position) below the threshold go to the lane's append buffer; full buffers merge into L
+        if constexpr (!HIST) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
+        }
         stream(
             [&](int sbits, int cpos, bool valid) {
-                const int32_t key = (int32_t)(((uint32_t)sbits & hmask) | (uint32_t)cpos);
-                // slot cnt (< 64: at most 32 buffered when a tile starts) is scratch until counted
+                // key: one bit-field insert (low ib bits from the wave-uniform position, the rest
+                // from the value); slot cnt (< 64: at most 32 buffered when a tile starts) is
+                // scratch until counted; the count takes the compare's carry in one add
+                // (positions past the stream end -- wave-uniform -- write nothing: the buffer rows
+                // past a lane's count must only hold keys that can never enter the list)
+                if (!valid) return;
+                int32_t key;
+                asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(key) : "v"(pmask_v), "s"(cpos), "v"(sbits));
                 sbuf[cnt * 64 + lane] = key;
-                cnt += (key < tau && valid) ? 1 : 0;
+                asm("v_cmp_lt_i32 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, 0, %0, vcc"
+                    : "+v"(cnt) : "v"(key), "v"(tau) : "vcc");
             },
             [&]() {
                 if (__any(cnt > 32)) compact_half();
             });
         while (__any(cnt > 0)) compact_half();
+        if (lane == 0) next = claim();
 
         // ---- finalize: certification and the list of candidates the FP64 re-rank must see
-        const float nmax_w = s_nmax;   // largest |c|^2 of the block's candidates
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < LS; ++i) sbuf[i * 64 + lane] = (uint32_t)L[i];
+        for (int i = 0; i < kprime; ++i) sbuf[i * 64 + lane] = (uint32_t)L[i];
+        // rows kprime..63 hold a byte table: the run of every stream position (runs < 256), so a
+        // kept candidate's column is two independent LDS reads away
+        constexpr int TBL = (64 - kprime) * 64 * 4;
+        uint8_t* const tbl = reinterpret_cast<uint8_t*>(sbuf + kprime * 64);
+        const bool use_tbl = ncand <= TBL;
+        auto run_of = [&](int pos) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_ro[mid] <= pos) lo = mid; else hi = mid - 1;
+            }
+            return lo;
+        };
+        if (use_tbl) {
+            for (int rr = 0; rr < nr; ++rr) {
+                const int e = s_ro[rr + 1];
+                for (int t = s_ro[rr] + lane; t < e; t += 64) tbl[t] = (uint8_t)rr;
+            }
+        }
         __syncthreads();
         if (qmine) {
             const int k = p.k;
@@ -485,16 +589,16 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
             if (ok) {
                 // the re-rank needs every kept candidate whose lower bound reaches U
+                // (the list is sorted and its screened distances nondecreasing, so the test
+                // holds for a prefix of the slots past kk: count it from the registers)
                 int r = kk;
-                while (r < nl && (double)key_dist((int32_t)sbuf[r * 64 + lane], hmask) - eps <= U) ++r;
+#pragma unroll
+                for (int j = 0; j < kprime; ++j)
+                    r += (j >= kk && j < nl && (double)key_dist(L[j], hmask) - eps <= U) ? 1 : 0;
                 for (int j = 0; j < r; ++j) {
                     const int32_t key = (int32_t)sbuf[j * 64 + lane];
                     const int pos = (int)(key & pmask);
-                    int lo = 0, hi = nr - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_ro[mid] <= pos) lo = mid; else hi = mid - 1;
-                    }
+                    const int lo = use_tbl ? (int)tbl[pos] : run_of(pos);
                     a.cand_col[(int64_t)j * a.n_q + q_mine] = s_rb[lo] + (pos - s_ro[lo]);
                     a.cand_d[(int64_t)j * a.n_q + q_mine] = key_dist(key, hmask);
                 }
@@ -506,6 +610,16 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 a.fallback[f] = q_mine;
             }
         }
+        };
+        // long candidate streams keep a deeper list: their keys carry more position bits (coarser
+        // distances) and their neighbour distances are closer together
+        if constexpr (LS == 64 && KPB < 64) {
+            if (ncand > a.large_ncand) {
+                body(std::integral_constant<int, 64>{});
+                continue;
+            }
+        }
+        body(std::integral_constant<int, KPB>{});
     }
 }
 
@@ -719,13 +833,19 @@ __global__ void k_mfma_probe(float* out) {
 
 template <int PREC, int KB>
 hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
-    if (a.kprime <= 32) {
-        if (a.hist) k_screen_knn<PREC, KB, 32, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
-        else k_screen_knn<PREC, KB, 32, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
-    } else {
-        if (a.hist) k_screen_knn<PREC, KB, 64, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
-        else k_screen_knn<PREC, KB, 64, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);
+#define IRC_SK(KPB)                                                                          \
+    do {                                                                                     \
+        if (a.hist) k_screen_knn<PREC, KB, KPB, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);   \
+        else k_screen_knn<PREC, KB, KPB, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);         \
+    } while (0)
+    switch (a.kprime) {
+        case 32: IRC_SK(32); break;
+        case 40: IRC_SK(40); break;
+        case 48: IRC_SK(48); break;
+        case 64: IRC_SK(64); break;
+        default: return hipErrorInvalidValue;
     }
+#undef IRC_SK
     return hipGetLastError();
 }
 
