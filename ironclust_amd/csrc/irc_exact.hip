@@ -229,43 +229,78 @@ __global__ void k_rho_cols(int64_t n_cols, const int* __restrict__ col_key, cons
 // the query's is the nearest denser spike, because kNN(a) is a prefix of C(a) in key
 // order; its canonical distance is recomputed here (one row pair per query).  Queries
 // without such an entry are appended to `unres`.
-__global__ void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_col,
-                                 const float* __restrict__ rho_col, float* __restrict__ delta_slot,
-                                 int64_t* __restrict__ nneigh_slot, int* __restrict__ unres,
-                                 int* __restrict__ n_unres) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nq) return;
-    const int qc = p.q_col[q];
+template <int NV>
+__global__ __launch_bounds__(64) void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_col,
+                                                       const float* __restrict__ rho_col,
+                                                       float* __restrict__ delta_slot,
+                                                       int64_t* __restrict__ nneigh_slot, int* __restrict__ unres,
+                                                       int* __restrict__ n_unres) {
+    // one wavefront per 64 consecutive queries; rows are gathered cooperatively into LDS
+    // (consecutive lanes fetch consecutive 16-byte pieces, global_load_lds) and read back per
+    // lane, so a load instruction touches a few rows instead of 64
+    __shared__ float4 stage[64 * NV];
+    const int l = threadIdx.x;
+    const int q = blockIdx.x * 64 + l;
+    const bool qv = q < nq;
+    const int qc = p.q_col[qv ? q : nq - 1];
     const float ra = rho_col[qc];
-    const int k = p.k;
-    for (int j = 0; j < k; ++j) {
-        const int col = knn_col[(int64_t)j * p.nq + q];
-        if (rho_col[col] > ra) {
-            const int slot = p.q_out[q];
-            const float4* a4 = reinterpret_cast<const float4*>(p.feat + (int64_t)qc * p.DP);
-            const float4* b4 = reinterpret_cast<const float4*>(p.feat + (int64_t)col * p.DP);
+    int found = -1;
+    if (qv) {
+        for (int j = 0; j < p.k; ++j) {
+            const int col = knn_col[(int64_t)j * p.nq + q];
+            if (rho_col[col] > ra) {
+                found = col;
+                break;
+            }
+        }
+    }
+    auto gather = [&](int col) {
+        int crow[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) crow[i] = __shfl(col, (i * 64 + l) / NV, 64);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int piece = i * 64 + l;
+            const int part = piece - (piece / NV) * NV;
+            const float* src = p.feat + (int64_t)crow[i] * p.DP + 4 * part;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(stage + i * 64), 16, 0, 0);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the rows have landed in LDS
+    };
+    gather(qc);
+    float4 a[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) a[i] = stage[l * NV + i];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // read back before the next gather lands
+    if (__any(found >= 0)) {
+        gather(found >= 0 ? found : qc);
+        if (found >= 0) {
             double t = 0.0;
-            for (int i = 0; i < p.DP / 4; ++i) {
-                const float4 x = a4[i], y = b4[i];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const float4 x = a[i], y = stage[l * NV + i];
                 double d;
                 d = (double)x.x - (double)y.x; t = fma(d, d, t);
                 d = (double)x.y - (double)y.y; t = fma(d, d, t);
                 d = (double)x.z - (double)y.z; t = fma(d, d, t);
                 d = (double)x.w - (double)y.w; t = fma(d, d, t);
             }
+            const int slot = p.q_out[q];
             const float dl = (float)(sqrt(t) * (double)ra);
             if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
                 delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
                 nneigh_slot[slot] = p.col_ret[qc];
             } else {
                 delta_slot[slot] = dl;
-                nneigh_slot[slot] = p.col_ret[col];
+                nneigh_slot[slot] = p.col_ret[found];
             }
-            return;
         }
     }
-    const int u = atomicAdd(n_unres, 1);
-    unres[u] = q;
+    if (qv && found < 0) {
+        const int u = atomicAdd(n_unres, 1);
+        unres[u] = q;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -512,8 +547,18 @@ hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const
                                  const float* rho_col, float* delta_slot, int64_t* nneigh_slot, int* unres,
                                  int* n_unres) {
     if (nq <= 0) return hipSuccess;
-    k_delta_from_knn<<<(nq + 255) / 256, 256, 0, st>>>(p, nq, knn_col, rho_col, delta_slot, nneigh_slot,
-                                                       unres, n_unres);
+    const unsigned grid = (unsigned)((nq + 63) / 64);
+#define IRC_DK(NV)                                                                                       \
+    case NV:                                                                                             \
+        k_delta_from_knn<NV><<<grid, 64, 0, st>>>(p, nq, knn_col, rho_col, delta_slot, nneigh_slot, unres, \
+                                                  n_unres);                                              \
+        break;
+    switch (p.DP / 4) {
+        IRC_DK(1) IRC_DK(2) IRC_DK(3) IRC_DK(4) IRC_DK(5) IRC_DK(6) IRC_DK(7) IRC_DK(8)
+        IRC_DK(9) IRC_DK(10) IRC_DK(11) IRC_DK(12) IRC_DK(13) IRC_DK(14) IRC_DK(15) IRC_DK(16)
+        default: return hipErrorInvalidValue;
+    }
+#undef IRC_DK
     return hipGetLastError();
 }
 
