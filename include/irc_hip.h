@@ -164,14 +164,15 @@ typedef struct {
  *   rho     float32(1/sqrt(d2_k))                    (vrRho before rho/max(rho), irc2.m:3669)
  *   delta   float32(sqrt(d2_min) * rho)              (vrDelta, irc2.m:3720-3725)
  *   nneigh  1-based spike id of the nearest denser spike, or self
- *   knn     k x n_spikes, 1-based spike ids, nearest first (may be NULL) (_knn_<site>.irc)
+ *   knn     k x n_spikes int32 (miKnn's type), 1-based spike ids, nearest first (may be NULL)
+ *           (_knn_<site>.irc)
  *   dk      sqrt(d2_k) (may be NULL)
  */
 typedef struct {
     float* rho;
     float* delta;
     int64_t* nneigh;
-    int64_t* knn;
+    int32_t* knn;
     float* dk;
 } irc_outputs;
 

@@ -802,7 +802,8 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
     CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
     float *o_rho, *o_delta, *o_dk = nullptr;
-    int64_t *o_nn, *o_knn = nullptr;
+    int64_t* o_nn;
+    int32_t* o_knn = nullptr;
     if (dev) {
         o_rho = out->rho;
         o_delta = out->delta;
@@ -817,8 +818,8 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         o_delta = ptr<float>(ctx->b_out_delta);
         o_nn = ptr<int64_t>(ctx->b_out_nn);
         if (out->knn) {
-            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int64_t) * (size_t)N * k));
-            o_knn = ptr<int64_t>(ctx->b_out_knn);
+            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int32_t) * (size_t)N * k));
+            o_knn = ptr<int32_t>(ctx->b_out_knn);
         }
         if (out->dk) {
             CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)N));
@@ -879,7 +880,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
                     &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
         HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
-                                      o_knn, nullptr));
+                                      nullptr, o_knn));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
         HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
         HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol), o_delta,
@@ -945,7 +946,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         HIPCHK(hipMemcpyAsync(out->rho, o_rho, sizeof(float) * N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(out->delta, o_delta, sizeof(float) * N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(out->nneigh, o_nn, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st));
-        if (out->knn) HIPCHK(hipMemcpyAsync(out->knn, o_knn, sizeof(int64_t) * (size_t)N * k, hipMemcpyDeviceToHost, st));
+        if (out->knn) HIPCHK(hipMemcpyAsync(out->knn, o_knn, sizeof(int32_t) * (size_t)N * k, hipMemcpyDeviceToHost, st));
         if (out->dk) HIPCHK(hipMemcpyAsync(out->dk, o_dk, sizeof(float) * N, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));

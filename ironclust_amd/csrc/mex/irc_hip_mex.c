@@ -167,8 +167,8 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         out.delta = (float*)mxGetData(plhs[1]);
         out.nneigh = (int64_t*)mxGetData(plhs[2]);
         if (nlhs > 3) {
-            plhs[3] = mxCreateNumericMatrix((mwSize)b.k, (mwSize)b.n_spikes, mxINT64_CLASS, mxREAL);
-            out.knn = (int64_t*)mxGetData(plhs[3]);
+            plhs[3] = mxCreateNumericMatrix((mwSize)b.k, (mwSize)b.n_spikes, mxINT32_CLASS, mxREAL);
+            out.knn = (int32_t*)mxGetData(plhs[3]);
         }
         check(irc_rho_delta_batch(ctx(), &b, &o, &out));
         mxFree(spk); mxFree(sp); mxFree(np); mxFree(lim); mxFree(lk);

@@ -57,7 +57,7 @@ class DriftProblem(ctypes.Structure):
 
 
 class Outputs(ctypes.Structure):
-    _fields_ = [("rho", _c_f32p), ("delta", _c_f32p), ("nneigh", _c_i64p), ("knn", _c_i64p), ("dk", _c_f32p)]
+    _fields_ = [("rho", _c_f32p), ("delta", _c_f32p), ("nneigh", _c_i64p), ("knn", _c_i32p), ("dk", _c_f32p)]
 
 
 class Stats(ctypes.Structure):
@@ -254,9 +254,9 @@ class Context:
         rho = np.zeros(N, np.float32)
         delta = np.zeros(N, np.float32)
         nn = np.zeros(N, np.int64)
-        knn = np.zeros((N, int(k)), np.int64) if want_knn else None
+        knn = np.zeros((N, int(k)), np.int32) if want_knn else None
         dk = np.zeros(N, np.float32) if want_dk else None
-        out = Outputs(_p(rho, _c_f32p), _p(delta, _c_f32p), _p(nn, _c_i64p), _p(knn, _c_i64p), _p(dk, _c_f32p))
+        out = Outputs(_p(rho, _c_f32p), _p(delta, _c_f32p), _p(nn, _c_i64p), _p(knn, _c_i32p), _p(dk, _c_f32p))
         o = opts or make_options()
         o.mem = MEM_HOST
         self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(out)))
@@ -286,7 +286,7 @@ class Context:
         o = opts or make_options()
         o.mem = MEM_DEVICE
         outs = Outputs(_dptr(out["rho"], _c_f32p), _dptr(out["delta"], _c_f32p), _dptr(out["nneigh"], _c_i64p),
-                       _dptr(out.get("knn"), _c_i64p), _dptr(out.get("dk"), _c_f32p))
+                       _dptr(out.get("knn"), _c_i32p), _dptr(out.get("dk"), _c_f32p))
         self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(outs)))
 
     # ---- legacy A/B and dc forms ------------------------------------------------------

@@ -85,7 +85,7 @@ def test_full_size_site_subset_bf16():
     out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
                delta=torch.empty(N, dtype=torch.float32, device="cuda"),
                nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
-               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+               knn=torch.empty((N, k), dtype=torch.int32, device="cuda"))
     ctx = irc2.get_ctx(0)
     ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
     torch.cuda.synchronize()
@@ -119,7 +119,7 @@ def test_graph_replay_matches_eager():
     out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
                delta=torch.empty(N, dtype=torch.float32, device="cuda"),
                nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
-               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+               knn=torch.empty((N, k), dtype=torch.int32, device="cuda"))
     ctx = libirc.Context(0)
     try:
         for feats_round in range(2):
