@@ -40,6 +40,9 @@ def parse():
     ap.add_argument("--per-site", type=int, default=0,
                     help="also time the per-site call pattern (irc_knn_drift + irc_delta_drift per site, host "
                          "buffers, the reference's rho_paged_site_/delta_paged_site_ loop) on this many sites")
+    ap.add_argument("--host", action="store_true",
+                    help="also time the host-buffer form of the headline call (features copied in and results "
+                         "copied out over PCIe inside the timed call)")
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -134,6 +137,18 @@ def main():
             forms[form]["graph_identical"] = all(torch.equal(out[kk], results[form][kk]) for kk in out)
         print(f"# {form}: {forms[form]}", file=sys.stderr, flush=True)
     head = a.forms.split(",")[0]
+    host_rate = None
+    if a.host:
+        import numpy as np
+        hl = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in dev.items()}
+        hopts = libirc.make_options(precision=PREC[head])
+        ctx.rho_delta_batch(hl, k, hopts, want_knn=True)
+        t3 = time.perf_counter()
+        hr = ctx.rho_delta_batch(hl, k, hopts, want_knn=True)
+        host_dt = time.perf_counter() - t3
+        assert np.array_equal(hr["rho"], results[head]["rho"].cpu().numpy())
+        host_rate = {"spikes_per_s": N / host_dt, "ms_per_call": host_dt * 1e3}
+        print(f"# host buffers ({head}): {host_rate}", file=sys.stderr, flush=True)
     # all forms must agree bit for bit
     for form in results:
         for kk in results[head]:
@@ -146,6 +161,7 @@ def main():
         assert bool((r["knn"][:, 0] == ids).all())
     hf = forms[head]
     flops = 2.0 * D * pairs
+    traffic = screen_traffic(a.workload, head)
     peak = PEAK[head]
     ach = flops / (hf["knn_kernel_ms"] * 1e-3)
     line = {
@@ -158,18 +174,39 @@ def main():
                    "form": {"bf16": "BF16 hi/lo MFMA screen + FP64 re-rank", "fp32": "FP32 MFMA screen + FP64 re-rank",
                             "exact": "FP64 exact"}[head]},
         "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": ach / peak, "traffic": None,
+                     "frac": ach / peak, "traffic": traffic,
                      "kernel": "kNN pass (screen kernel when screened)",
-                     "flops_per_launch": flops},
+                     "flops_per_launch": flops,
+                     "traffic_source": TRAFFIC_SRC if traffic is not None else None},
         "forms": forms,
     }
     if tuned:
         line["autotune"] = tuned
+    if host_rate:
+        line["host_buffers_pcie_inclusive"] = host_rate
     if a.per_site > 0:
         line["per_site_launches"] = per_site(ds, lay, a, ctx, PREC[head], results[head])
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ds, a)
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_SRC = ("profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch of the screen "
+               "kernel, each scaled by the factor a plain device copy of known size reads")
+
+
+def screen_traffic(workload, form):
+    """HBM bytes per launch of the form's screen kernel from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if form not in ("bf16", "fp32") or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    prefix = "k_screen_knn<1," if form == "bf16" else "k_screen_knn<0,"
+    for name, d in t.get("forms", {}).get(form, {}).items():
+        if name.startswith(prefix):
+            return d["hbm_bytes"]
+    return None
 
 
 def per_site(ds, lay, a, ctx, prec, ref):
