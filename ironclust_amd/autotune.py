@@ -18,11 +18,12 @@ GRID = {
     libirc.PREC_BF16_SCREEN: [
         dict(k_over=48), dict(k_over=40), dict(k_over=64),
         dict(k_over=48, waves_per_cu=16), dict(k_over=48, flags=2),
-        dict(k_over=48, large_ncand=4096), dict(k_over=48, tile=8),
+        dict(k_over=48, large_ncand=4096), dict(k_over=48, tile=8), dict(k_over=48, tile=16),
+        dict(k_over=48, tile=24), dict(k_over=40, tile=16),
     ],
     libirc.PREC_FP32: [
         dict(k_over=40), dict(k_over=32), dict(k_over=48),
-        dict(k_over=40, waves_per_cu=16), dict(k_over=40, tile=8),
+        dict(k_over=40, waves_per_cu=16), dict(k_over=40, tile=8), dict(k_over=40, tile=16),
     ],
 }
 

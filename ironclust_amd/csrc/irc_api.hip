@@ -341,9 +341,11 @@ struct DevWork {
 // kNN pass: MFMA screen + certification + FP64 re-rank for every block that fits the
 // screen, the exact FP64 kernel for the rest and for uncertified queries.  The work list comes
 // from `ranges` (host planning, per-site entry points) or is built on the device (`dw`, batch).
+// feat_out: when set, the [n_cols][DP] FP32 rows (p.feat) are written here from F (stride D) --
+// by the screen's packing kernel in the same pass over F, or by the FP32 packer on the exact path
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
                    int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
-                   double* dk2) {
+                   double* dk2, float* feat_out = nullptr) {
     hipStream_t st = ctx->stream;
     int64_t pairs = 0;
     if (ranges)
@@ -359,6 +361,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     // wider features take the exact path
     const bool screen = (prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN) && irc::screen_kp(p.D) <= 64;
     if (!screen) {
+        if (feat_out) HIPCHK(irc::launch_pack_fp32(st, n_cols, p.D, p.DP, F, feat_out));
         ctx->stats.precision_used = IRC_PREC_EXACT;
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
         HIPCHK(irc::launch_knn_exact(st, p, nullptr, nq, persistent_grid(ctx, 8), knn_col, dk2, nullptr));
@@ -456,7 +459,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->b_cand_eps, sizeof(double) * (size_t)nq));
     HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
-                                   ctx->b_scr[6].p, ctx->b_scr[7].p, ptr<float>(ctx->b_scr[2])));
+                                   ctx->b_scr[6].p, ctx->b_scr[7].p, ptr<float>(ctx->b_scr[2]), feat_out, p.DP));
     float* run_nmax = ptr<float>(ctx->b_scr[2]) + n_cols;
     HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->b_scr[2]),
                                 run_nmax));
@@ -851,7 +854,6 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     ctx->stats.n_queries = N;
     HIPCHK(irc::launch_fill_int(st, d_flags, 3, 0));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(irc::launch_pack_fp32(st, n_cols, D, DP, dF, ptr<float>(ctx->b_feat)));
     HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
                                   ptr<int>(ctx->b_runl)));
     HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
@@ -878,7 +880,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
         CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
-                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
+                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), ptr<float>(ctx->b_feat)));
         HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
                                       nullptr, o_knn));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));

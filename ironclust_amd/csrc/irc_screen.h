@@ -54,8 +54,10 @@ enum { SCREEN_FP32 = 0, SCREEN_BF16 = 1 };
 
 int screen_kp(int D);                                  // padded operand width (multiple of 16)
 double screen_eta(int prec, int D, int KP);            // rigorous error factor
+// feat (optional): also write the [n_cols][DP] zero-padded FP32 rows (F must have stride D)
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, void* qhi, void* qlo, float* norm);
+                              void* hi, void* lo, void* qhi, void* qlo, float* norm, float* feat = nullptr,
+                              int DP = 0);
 // seg_ctr: 8 device counters (zeroed by the launch); seg_beg: 9 device offsets of the
 // per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,

@@ -51,7 +51,8 @@ constexpr int PACK_COLS = 32;
 
 __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
                               void* __restrict__ hi, void* __restrict__ lo, void* __restrict__ qhi,
-                              void* __restrict__ qlo, float* __restrict__ norm) {
+                              void* __restrict__ qlo, float* __restrict__ norm, float* __restrict__ feat,
+                              int DP) {
     __shared__ float srow[PACK_COLS * IRC_DMAX];
     __shared__ float snorm[PACK_COLS];
     const int nch = KP / 8;
@@ -73,6 +74,17 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
         norm[col0 + threadIdx.x] = n;
     }
     __syncthreads();
+    if (feat) {
+        // the FP32 rows of these columns, zero padded to DP: contiguous float4 stores
+        const int n4 = DP / 4;
+        float4* f4 = reinterpret_cast<float4*>(feat + col0 * DP);
+        for (int e = threadIdx.x; e < ncol * n4; e += blockDim.x) {
+            const int cc = e / n4, d0 = 4 * (e - cc * n4);
+            const float* x = srow + cc * IRC_DMAX;
+            f4[e] = make_float4(d0 < D ? x[d0] : 0.0f, d0 + 1 < D ? x[d0 + 1] : 0.0f, d0 + 2 < D ? x[d0 + 2] : 0.0f,
+                                d0 + 3 < D ? x[d0 + 3] : 0.0f);
+        }
+    }
     const int c = threadIdx.x / nch, j = threadIdx.x - c * nch;
     if (c >= ncol) return;
     const int64_t col = col0 + c;
@@ -868,11 +880,12 @@ double screen_eta(int prec, int D, int KP) {
 }
 
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, void* qhi, void* qlo, float* norm) {
+                              void* hi, void* lo, void* qhi, void* qlo, float* norm, float* feat, int DP) {
+    if (feat && FS != D) return hipErrorInvalidValue;
     if (n_cols <= 0) return hipSuccess;
     const int64_t blocks = (n_cols + PACK_COLS - 1) / PACK_COLS;
     k_screen_pack<<<(unsigned)blocks, PACK_COLS * (KP / 8), 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
-                                                                     norm);
+                                                                     norm, feat, DP);
     return hipGetLastError();
 }
 

@@ -40,7 +40,8 @@ def _opts(P):
         prec = {"exact": libirc.PREC_EXACT, "fp32": libirc.PREC_FP32, "bf16": libirc.PREC_BF16_SCREEN,
                 "auto": libirc.PREC_AUTO}[prec]
     return libirc.make_options(precision=prec, k_over=int(P.get("k_over", 0)),
-                               force_fail=bool(P.get("force_fail", False)), flags=int(P.get("flags", 0)))
+                               force_fail=bool(P.get("force_fail", False)), flags=int(P.get("flags", 0)),
+                               large_ncand=int(P.get("large_ncand", 0)))
 
 
 def search_knn_drift_(vl_in, mrFet_out, viDrift_out, mlDrift1, P=None, viSpk_out=None):

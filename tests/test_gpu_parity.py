@@ -72,6 +72,21 @@ def test_batch_large_blocks_screened(prec, ctx):
     assert st["n_screen_fallback"] < 0.01 * ds["n_spikes"], st
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("large_ncand", [100, 30000])
+def test_screen_list_depth_paths(prec, large_ncand):
+    # blocks of 1k-22k candidates: large_ncand = 100 sends every block through the 64-deep list
+    # body, 30000 keeps the 40/48-deep body past the finalize's byte table (binary-search lookup)
+    ds = synth.make_dataset(45000, 2, 16, 6, 6, k=30, drift_step=0.2, n_links=3, band=2, seed=39)
+    lay = irc2.site_layout(ds)
+    ref = O.sort_batch(ds)
+    rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": 30, "precision": prec, "large_ncand": large_ncand})
+    np.testing.assert_array_equal(knn.T, ref["knn"])
+    np.testing.assert_array_equal(rho, ref["rho"])
+    np.testing.assert_array_equal(nn, ref["nneigh"])
+    np.testing.assert_array_equal(delta, ref["delta"])
+
+
 def test_full_size_site_subset_bf16():
     """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
     oracle recomputes three whole sites (their results do not depend on other sites)."""
