@@ -114,3 +114,26 @@ def subset_sites(ds, sites):
                F1=np.ascontiguousarray(F1[orig - 1]), F2=None, site1=s1[orig - 1].astype(np.int32),
                site2=None, lim=sub_lim.astype(np.int64), link=np.asarray(ds["link"]))
     return sub, orig
+
+
+def write_detect_files(ds, prefix, nPc, nS, n_loads=2):
+    """Write a synthetic dataset as irc2 detect files (_fet_/_fet2_ per load, D = nPc x nS with
+    every feature selected); returns (ccviSpk_site_load, ccviSpk_site2_load, dimm_fet, mlPc)."""
+    from ironclust_amd import irc_files as IF
+    N = ds["n_spikes"]
+    F1 = ds["F1"].cpu().numpy().reshape(N, nPc, nS, order="F").transpose(1, 2, 0)
+    s1 = ds["site1"].cpu().numpy()
+    has2 = ds.get("F2") is not None
+    F2 = ds["F2"].cpu().numpy().reshape(N, nPc, nS, order="F").transpose(1, 2, 0) if has2 else None
+    s2 = ds["site2"].cpu().numpy() if has2 else None
+    cut = np.linspace(0, N, n_loads + 1).astype(np.int64)
+    c1, c2 = [], []
+    for i in range(n_loads):
+        a, b = cut[i], cut[i + 1]
+        x, y, _ = IF.detect_paged_save_(prefix, i + 1, F1[:, :, a:b], s1[a:b], ds["n_sites"],
+                                        F2[:, :, a:b] if has2 else None, s2[a:b] if has2 else None)
+        c1.append(x)
+        c2.append(y)
+    vn = np.diff(cut)
+    ml, _ = IF.get_mlPc_((nPc, nS, N), nC_max=nPc * nS)
+    return IF.merge_site_lists_(c1, vn), (IF.merge_site_lists_(c2, vn) if has2 else None), (nPc, nS, N), ml

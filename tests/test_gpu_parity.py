@@ -87,6 +87,27 @@ def test_screen_list_depth_paths(prec, large_ncand):
     np.testing.assert_array_equal(delta, ref["delta"])
 
 
+def test_batch_from_detect_files(tmp_path):
+    # a real irc2 run's hand-over: _fet_/_fet2_ files of two loads -> site-grouped columns -> one
+    # batch call -> per-site _knn_ files; results equal the oracle's on the same spikes
+    from ironclust_amd import irc_files as IF
+    from tests.helpers import write_detect_files
+    ds = synth.make_dataset(4000, 6, 12, 5, 3, k=8, drift_step=0.3, n_links=2, band=2, seed=42, f2=True)
+    prefix = str(tmp_path / "rec")
+    cc1, cc2, dimm, ml = write_detect_files(ds, prefix, 3, 4)
+    lay = IF.layout_from_fet_files(prefix, cc1, cc2, dimm, ml, ds["lim"], ds["link"], ds["n_spikes"], ds["k"])
+    ref = O.sort_batch(ds)
+    rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": ds["k"], "precision": "bf16"})
+    np.testing.assert_array_equal(knn.T, ref["knn"])
+    np.testing.assert_array_equal(rho, ref["rho"])
+    np.testing.assert_array_equal(nn, ref["nneigh"])
+    np.testing.assert_array_equal(delta, ref["delta"])
+    IF.save_knn_by_site(prefix, lay, knn)
+    for s in range(ds["n_sites"]):
+        ids = lay["col_spike"][lay["site_ptr"][s]:lay["site_ptr"][s] + lay["site_n_primary"][s]]
+        np.testing.assert_array_equal(IF.load_miKnn_site_(prefix, s + 1, ds["k"]), ref["knn"][ids - 1].T)
+
+
 def test_full_size_site_subset_bf16():
     """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
     oracle recomputes three whole sites (their results do not depend on other sites)."""
