@@ -17,6 +17,9 @@
  *   irc_delta_knn_ab    cuda_delta_knn_     irc.m:24519-24553        (kernel cuda_delta_knn.cu:19-88)
  *   irc_rho_drift_dc    cuda_rho_drift_     irc.m:24612-24637        (kernel irc_cuda_rho_drift.cu:28-108)
  *   irc_delta_drift_dc  cuda_delta_drift_   irc.m:24641-24666        (kernel irc_cuda_delta_drift.cu:28-132)
+ *   irc_calc_drift      calc_drift_         matlab/irc2.m:6373-6447  (drift linkage, the path's input)
+ *   irc_assign_chain    assignCluster_knn_  matlab/irc2.m:6681-6692  (cluster labels along nneigh, the
+ *                                            path's consumer)
  *
  * Results are exact to the canonical definition (FP64 distances accumulated with
  * fused multiply-add in dimension order; ties broken by lower spike id), whatever
@@ -254,6 +257,15 @@ typedef struct {
 
 irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_drift_out, int64_t* lim_out,
                           uint8_t* link_out, int32_t* counts_out);
+
+/* Cluster labels along the nearest-denser chains: the final loop of assignCluster_knn_
+ * (matlab/irc2.m:6681-6692), which visits spikes by descending rho and copies the cluster of
+ * nneigh.  labels[n] (in/out): the cluster of each peak (> 0) and 0 elsewhere on entry; on return
+ * every spike carries the cluster of the first labelled spike on its chain a, nneigh(a), ...
+ * nneigh[n]: 1-based (the batch call's nneigh output).  opt->mem selects host or device
+ * pointers (opt may be NULL: host). */
+irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int32_t* labels,
+                            const irc_options* opt);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(LIBDIR, "libirc_hip.so")
-SOURCES = ["irc_api.hip", "irc_exact.hip", "irc_screen.hip", "irc_legacy.hip", "irc_plan.hip", "irc_drift.hip"]
+SOURCES = ["irc_api.hip", "irc_exact.hip", "irc_screen.hip", "irc_legacy.hip", "irc_plan.hip", "irc_drift.hip",
+           "irc_assign.hip"]
 ARCH = "gfx950"
 
 

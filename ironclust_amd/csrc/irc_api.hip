@@ -17,6 +17,7 @@
 
 #include "irc_common.h"
 #include "irc_kernels.h"
+#include "irc_assign.h"
 #include "irc_drift.h"
 #include "irc_plan.h"
 #include "irc_screen.h"
@@ -49,6 +50,7 @@ struct irc_ctx {
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
     DevBuf b_plan;                                         // device-side work planning
     DevBuf b_drift;                                        // drift linkage scratch
+    DevBuf b_assign;                                       // chain labelling scratch
     DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
                           // re-rank, [2] pairs, [3] queries whose denser neighbour was outside the kNN list
     bool dstat_screen = false;  // [0], [1] hold this call's screen counters
@@ -617,7 +619,7 @@ void irc_ctx_destroy(irc_ctx* ctx) {
                      &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
                      &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
                      &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan, &ctx->b_dstat,
-                     &ctx->b_drift};
+                     &ctx->b_drift, &ctx->b_assign};
     for (DevBuf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->b_scr)
@@ -1145,6 +1147,37 @@ irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_
     HIPCHK(hipStreamSynchronize(st));
     memcpy(lim_out, lim.data(), sizeof(int64_t) * lim.size());
     *n_drift_out = C;
+    ctx->stats = irc_stats{};
+    ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
+    return IRC_OK;
+}
+
+irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int32_t* labels,
+                            const irc_options* opt) {
+    if (!ctx) return IRC_E_ARG;
+    if (n < 0 || (n > 0 && (!nneigh || !labels))) return fail(ctx, IRC_E_ARG, "null pointer in irc_assign_chain");
+    if (n >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "n must be < 2^31");
+    if (n == 0) return IRC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const bool dev = opt && opt->mem == IRC_MEM_DEVICE;
+    // scratch: jump targets, and the host form's copies of nneigh and the labels
+    const size_t o_nxt = 0, o_lab = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+    const size_t o_nn = (o_lab + sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+    CHK(ensure(ctx, ctx->b_assign, dev ? o_lab : o_nn + sizeof(int64_t) * (size_t)n));
+    char* base = reinterpret_cast<char*>(ctx->b_assign.p);
+    int32_t* d_nxt = reinterpret_cast<int32_t*>(base + o_nxt);
+    int32_t* d_lab = dev ? labels : reinterpret_cast<int32_t*>(base + o_lab);
+    const int64_t* d_nn = dev ? nneigh : reinterpret_cast<const int64_t*>(base + o_nn);
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(base + o_nn, nneigh, sizeof(int64_t) * (size_t)n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_lab, labels, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(irc::launch_assign_chain(st, n, d_nn, d_lab, d_nxt));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    if (!dev) HIPCHK(hipMemcpyAsync(labels, d_lab, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     ctx->stats = irc_stats{};
     ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
     return IRC_OK;

@@ -1,0 +1,13 @@
+// irc_assign.h -- density-peak label propagation along nneigh chains (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace irc {
+
+// lab[a] (in/out): cluster of a (> 0) or 0 (unassigned); nxt: scratch of n ints.  On return every
+// spike whose nneigh chain reaches a labelled spike carries that spike's label (the first labelled
+// spike on the chain a, nneigh(a), nneigh(nneigh(a)), ...).  nneigh is 1-based.
+hipError_t launch_assign_chain(hipStream_t st, int64_t n, const int64_t* nneigh, int32_t* lab, int32_t* nxt);
+
+}  // namespace irc

@@ -193,3 +193,84 @@ def plan_sort_page_(nTime_drift, viLim_drift, nTime_max_drift=256):
         si[p - 1] = (lim[ia - 1], lim[ib] - 1)
         so[p - 1] = (lim[iA - 1], lim[iB] - 1)
     return so, si, do, di
+
+
+# ---- density-peak assignment (the consumer of rho, delta, nneigh and the kNN) ---------------
+
+def _merge_overlap_groups(groups, n_repeat=10):
+    """cell2map_ (matlab/irc2.m:6754-6803) on overlap lists (1-based member sets): up to n_repeat
+    rounds in which every list with more than one member is replaced, for all its members, by the
+    union of its members' lists (later lists overwrite earlier ones); each entry then maps to its
+    smallest member (itself when its list is empty) and the map is compacted.  Returns
+    (cluster of each entry, 1-based, and the sorted representatives)."""
+    cur = [frozenset(g) for g in groups]
+    n = len(cur)
+    for _ in range(n_repeat):
+        nxt = [frozenset()] * n
+        for i in range(n):
+            if len(cur[i]) < 2:
+                continue
+            grown = set(cur[i])
+            for m in cur[i]:
+                grown |= cur[m - 1]
+            g = frozenset(grown)
+            for m in g:
+                nxt[m - 1] = g
+        if all(len(a) == len(b) for a, b in zip(nxt, cur)):
+            break
+        cur = nxt
+    rep = np.array([min(g) if g else i + 1 for i, g in enumerate(cur)], dtype=np.int64)
+    reps = np.unique(rep)
+    ids = np.searchsorted(reps, rep) + 1
+    return ids.astype(np.int32), reps
+
+
+def assignCluster_knn_(S_clu, P=None, miKnn=None):
+    """[viClu_spk, viSpk_peak, nClu_pre] = assignCluster_knn_(S_clu, viSite_spk, P)
+    (matlab/irc2.m:6640-6694).  S_clu: dict with rho, delta, nneigh (1-based); miKnn: k x N kNN
+    (1-based spike ids; only the peaks' columns are read).  Peaks (delta >= delta_cut, by rho
+    descending) whose kNN lists share a spike are merged on the host; the chain loop runs on the
+    GPU (irc_assign_chain)."""
+    P = P or {}
+    rho = np.asarray(S_clu["rho"])
+    delta = np.asarray(S_clu["delta"])
+    nneigh = np.asarray(S_clu["nneigh"], dtype=np.int64)
+    N = rho.size
+    ordrho = np.argsort(-rho.astype(np.float64), kind="stable") + 1
+    peaks = np.flatnonzero(delta >= float(P.get("delta_cut", 1.0))) + 1
+    n_pre = len(peaks)
+    peaks = peaks[np.argsort(-rho[peaks - 1].astype(np.float64), kind="stable")]
+    if len(peaks):
+        K = np.asarray(miKnn)[:, peaks - 1]
+        # spike -> peaks whose kNN list holds it; a peak's overlap list is the union over its spikes
+        owners = {}
+        for j in range(K.shape[1]):
+            for s in set(K[:, j].tolist()):
+                owners.setdefault(s, []).append(j + 1)
+        groups = [set().union(*(owners[s] for s in set(K[:, j].tolist()))) for j in range(K.shape[1])]
+        clu_peak, reps = _merge_overlap_groups(groups)
+        peaks_kept = peaks[reps - 1]
+    else:
+        peaks_kept = peaks
+    if len(peaks_kept) <= 1:
+        return np.ones(N, np.int32), ordrho[:1], n_pre
+    lab = np.zeros(N, np.int32)
+    lab[peaks - 1] = clu_peak
+    lab = get_ctx(P.get("device", 0)).assign_chain(nneigh, lab)
+    return lab, peaks_kept, n_pre
+
+
+def postCluster_(S_clu, P=None, miKnn=None):
+    """viClu and icl after assignCluster_knn_ and dpclus_remove_count_ (matlab/irc2.m:6520-6534,
+    6735-6749): clusters with fewer than P['min_count'] spikes are dropped and the ids compacted."""
+    P = P or {}
+    clu, icl, _ = assignCluster_knn_(S_clu, P, miKnn)
+    min_count = int(P.get("min_count", 0))
+    n = len(icl)
+    counts = np.bincount(clu[clu > 0], minlength=n + 1)[1:n + 1]
+    keep = counts >= min_count
+    if keep.all():
+        return clu, icl
+    new_id = np.zeros(n + 1, np.int32)
+    new_id[1:][keep] = np.arange(1, int(keep.sum()) + 1)
+    return np.where(clu > 0, new_id[np.clip(clu, 0, n)], 0).astype(np.int32), icl[keep]

@@ -20,7 +20,7 @@ EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
     "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
-    "irc_calc_drift",
+    "irc_calc_drift", "irc_assign_chain",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -127,6 +127,8 @@ def load(path=None):
     lib.irc_calc_drift.argtypes = [ctypes.c_void_p, ctypes.POINTER(DriftProblem), _c_i32p, _c_i64p,
                                    ctypes.POINTER(ctypes.c_uint8), _c_i32p]
     lib.irc_calc_drift.restype = ctypes.c_int
+    lib.irc_assign_chain.argtypes = [ctypes.c_void_p, ctypes.c_int64, _c_i64p, _c_i32p, ctypes.POINTER(Options)]
+    lib.irc_assign_chain.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -344,6 +346,21 @@ class Context:
         if counts is not None:
             counts = np.ascontiguousarray(counts[:C].T) if C > 1 else None
         return link, lim, counts
+
+    def assign_chain(self, nneigh, labels):
+        """Cluster of every spike from the peaks' clusters along the nneigh chains (the last loop
+        of assignCluster_knn_).  nneigh: 1-based; labels: > 0 at peaks, 0 elsewhere.  Host numpy
+        arrays, or torch device tensors (int64 nneigh, int32 labels, updated in place)."""
+        if hasattr(labels, "data_ptr"):
+            o = make_options()
+            o.mem = MEM_DEVICE
+            self._check(self.lib.irc_assign_chain(self.h, labels.numel(), _dptr(nneigh, _c_i64p),
+                                                  _dptr(labels, _c_i32p), ctypes.byref(o)))
+            return labels
+        nn = np.ascontiguousarray(nneigh, dtype=np.int64)
+        lab = np.array(labels, dtype=np.int32, copy=True)
+        self._check(self.lib.irc_assign_chain(self.h, lab.size, _p(nn, _c_i64p), _p(lab, _c_i32p), None))
+        return lab
 
     def delta_drift_dc(self, n1, miSite12, fet1, fet2, rho12, dc2):
         m = np.asfortranarray(np.asarray(miSite12, dtype=np.int32))

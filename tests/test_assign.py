@@ -1,0 +1,65 @@
+"""Density-peak assignment oracle (assignCluster_knn_, cell2map_, dpclus_remove_count_), pinned
+by hand-computed cases and independent chain walks (CPU)."""
+import numpy as np
+
+from oracle import assign as A
+from oracle import oracle as O
+from ironclust_amd import synth
+
+
+def test_cell2map_hand_cases():
+    # overlap lists (self-inclusive): {1,2,3} chain together, 4 alone, {5,6}
+    m, reps = A.cell2map([[1, 2], [1, 2, 3], [2, 3], [4], [5, 6], [5, 6]])
+    assert list(m) == [1, 1, 1, 2, 3, 3] and list(reps) == [1, 4, 5]
+    # a path 1-2-3-4-5, traced by hand through the reference's update order (later lists overwrite
+    # earlier ones within a round): round 1 leaves {1..5}, {2..5}, {3,4,5} x 3, round 2 changes no
+    # size, so the path splits into three clusters -- [1, 2, 3, 3, 3]
+    path = [[1, 2], [1, 2, 3], [2, 3, 4], [3, 4, 5], [4, 5]]
+    m, reps = A.cell2map(path)
+    assert list(m) == [1, 2, 3, 3, 3] and list(reps) == [1, 2, 3]
+    m, reps = A.cell2map([[1], [2], [3]])
+    assert list(m) == [1, 2, 3] and list(reps) == [1, 2, 3]
+    # a mutual pair and a star: {1,3} and {2,3} both reach 3 -> one cluster
+    m, reps = A.cell2map([[1, 3], [2, 3], [1, 2, 3]])
+    assert list(m) == [1, 1, 1] and list(reps) == [1]
+
+
+def _chain_walk_labels(lab0, nneigh):
+    """Independent restatement: the label of the first labelled spike on a -> nneigh(a) -> ..."""
+    out = lab0.copy()
+    for a in range(len(lab0)):
+        x, seen = a, 0
+        while out[x] == 0 and lab0[x] == 0 and nneigh[x] - 1 != x and seen <= len(lab0):
+            x = nneigh[x] - 1
+            seen += 1
+        out[a] = lab0[x] if lab0[x] > 0 else out[x]
+    return out
+
+
+def test_assign_follows_chains_and_merges_overlapping_peaks():
+    ds = synth.make_dataset(4000, 3, 6, 4, 3, k=10, drift_step=0.3, n_links=2, band=2, seed=51)
+    r = O.sort_batch(ds)
+    rho, delta, nn, knn = r["rho"], r["delta"], r["nneigh"], r["knn"].T
+    for cut in (np.quantile(delta, 0.99), np.quantile(delta, 0.999)):
+        clu, peaks, n_pre = A.assign_cluster_knn(rho, delta, nn, knn, delta_cut=cut)
+        assert n_pre == int((delta >= cut).sum())
+        assert clu.min() >= 1
+        # peaks: every original peak carries the cluster its kNN-overlap group maps to
+        pk = np.flatnonzero(delta >= cut) + 1
+        lab0 = np.zeros(len(rho), np.int32)
+        lab0[pk - 1] = clu[pk - 1]
+        np.testing.assert_array_equal(clu, _chain_walk_labels(lab0, nn))
+        # two peaks whose kNN lists share a spike are in one cluster
+        for i in pk[:40]:
+            for j in pk[:40]:
+                if set(knn[:, i - 1]) & set(knn[:, j - 1]):
+                    assert clu[i - 1] == clu[j - 1]
+        assert len(peaks) == clu.max()
+
+
+def test_remove_count():
+    clu = np.array([1, 1, 2, 3, 3, 3, 0], np.int32)
+    out, icl = A.remove_count(clu, np.array([10, 20, 30]), 2)
+    assert list(out) == [1, 1, 0, 2, 2, 2, 0] and list(icl) == [10, 30]
+    out, icl = A.remove_count(clu, np.array([10, 20, 30]), 1)
+    assert list(out) == list(clu)

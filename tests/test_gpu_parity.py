@@ -108,6 +108,32 @@ def test_batch_from_detect_files(tmp_path):
         np.testing.assert_array_equal(IF.load_miKnn_site_(prefix, s + 1, ds["k"]), ref["knn"][ids - 1].T)
 
 
+def test_assignment_matches_oracle():
+    # assignCluster_knn_ on the batch outputs: host merge of overlapping peaks + GPU chain labels,
+    # against the oracle's sequential restatement; the device-pointer form agrees with the host one
+    import torch
+    from oracle import assign as A
+    ds = synth.make_dataset(60000, 8, 12, 10, 6, k=20, drift_step=0.3, n_links=3, band=2, seed=52)
+    lay = irc2.site_layout(ds)
+    rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": 20, "precision": "bf16"})
+    for q in (0.99, 0.999):
+        cut = float(np.quantile(delta, q))
+        w_clu, w_peaks, w_pre = A.assign_cluster_knn(rho, delta, nn, knn, delta_cut=cut)
+        g_clu, g_peaks, g_pre = irc2.assignCluster_knn_({"rho": rho, "delta": delta, "nneigh": nn},
+                                                        {"delta_cut": cut}, knn)
+        assert g_pre == w_pre
+        np.testing.assert_array_equal(g_peaks, w_peaks)
+        np.testing.assert_array_equal(g_clu, w_clu)
+    lab0 = np.zeros(len(rho), np.int32)
+    pk = np.flatnonzero(delta >= cut)
+    lab0[pk] = np.arange(1, len(pk) + 1, dtype=np.int32)
+    ctx = irc2.get_ctx(0)
+    host = ctx.assign_chain(nn, lab0)
+    dev = ctx.assign_chain(torch.as_tensor(nn, dtype=torch.int64).cuda(), torch.as_tensor(lab0).cuda())
+    np.testing.assert_array_equal(dev.cpu().numpy(), host)
+    assert host.min() >= 1
+
+
 def test_full_size_site_subset_bf16():
     """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
     oracle recomputes three whole sites (their results do not depend on other sites)."""
