@@ -20,6 +20,7 @@
  *   irc_calc_drift      calc_drift_         matlab/irc2.m:6373-6447  (drift linkage, the path's input)
  *   irc_assign_chain    assignCluster_knn_  matlab/irc2.m:6681-6692  (cluster labels along nneigh, the
  *                                            path's consumer)
+ *   irc_isolation_counts isolation_score_   matlab/irc2.m:6699-6731  (kNN-graph merges, knn_overlap_merge_)
  *
  * Results are exact to the canonical definition (FP64 distances accumulated with
  * fused multiply-add in dimension order; ties broken by lower spike id), whatever
@@ -266,6 +267,14 @@ irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_
  * pointers (opt may be NULL: host). */
 irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int32_t* labels,
                             const irc_options* opt);
+
+/* Neighbour-cluster counts behind isolation_score_ (matlab/irc2.m:6699-6731), used by
+ * knn_overlap_merge_ (matlab/irc2.m:2156-2174): counts[(c-1)*(n_clu+1) + d] = how many kNN entries
+ * of the spikes of cluster c (1..n_clu) lie in cluster d (0: in no cluster).  knn: k x n int32
+ * 1-based spike ids (the batch call's output); clu[n]: cluster per spike (<= 0: none).
+ * counts (n_clu x (n_clu+1), uint64) is overwritten.  n_clu <= 4096.  opt->mem: host or device. */
+irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int32_t* knn, const int32_t* clu,
+                                int32_t n_clu, uint64_t* counts, const irc_options* opt);
 
 #ifdef __cplusplus
 }

@@ -1183,4 +1183,38 @@ irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int3
     return IRC_OK;
 }
 
+irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int32_t* knn, const int32_t* clu,
+                                int32_t n_clu, uint64_t* counts, const irc_options* opt) {
+    if (!ctx) return IRC_E_ARG;
+    if (n < 0 || k < 1 || n_clu < 1 || !counts || (n > 0 && (!knn || !clu)))
+        return fail(ctx, IRC_E_ARG, "invalid arguments in irc_isolation_counts");
+    if (n_clu > 4096) return fail(ctx, IRC_E_UNSUPPORTED, "n_clu=%d exceeds 4096", n_clu);
+    if (n >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "n must be < 2^31");
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const bool dev = opt && opt->mem == IRC_MEM_DEVICE;
+    const size_t nc = (size_t)n_clu * (size_t)(n_clu + 1);
+    const size_t o_cnt = 0, o_clu = (sizeof(uint64_t) * nc + 255) & ~(size_t)255;
+    const size_t o_knn = (o_clu + sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+    CHK(ensure(ctx, ctx->b_assign, dev ? o_clu : o_knn + sizeof(int32_t) * (size_t)n * k));
+    char* base = reinterpret_cast<char*>(ctx->b_assign.p);
+    unsigned long long* d_cnt = dev ? reinterpret_cast<unsigned long long*>(counts)
+                                    : reinterpret_cast<unsigned long long*>(base + o_cnt);
+    const int32_t* d_clu = dev ? clu : reinterpret_cast<const int32_t*>(base + o_clu);
+    const int32_t* d_knn = dev ? knn : reinterpret_cast<const int32_t*>(base + o_knn);
+    if (!dev) {
+        HIPCHK(hipMemcpyAsync(base + o_clu, clu, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(base + o_knn, knn, sizeof(int32_t) * (size_t)n * k, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(uint64_t) * nc, st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    HIPCHK(irc::launch_isolation_counts(st, n, k, d_knn, d_clu, n_clu, d_cnt));
+    HIPCHK(hipEventRecord(ctx->ev[1], st));
+    if (!dev) HIPCHK(hipMemcpyAsync(counts, d_cnt, sizeof(uint64_t) * nc, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->stats = irc_stats{};
+    ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
+    return IRC_OK;
+}
+
 }  // extern "C"

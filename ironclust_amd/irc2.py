@@ -274,3 +274,36 @@ def postCluster_(S_clu, P=None, miKnn=None):
     new_id = np.zeros(n + 1, np.int32)
     new_id[1:][keep] = np.arange(1, int(keep.sum()) + 1)
     return np.where(clu > 0, new_id[np.clip(clu, 0, n)], 0).astype(np.int32), icl[keep]
+
+
+def isolation_score_(viClu, miKnn, nClu=None, P=None):
+    """[vrScore_clu, viClu_nn_clu] = isolation_score_(S_auto, miKnn_spk) (matlab/irc2.m:6699-6731)
+    from the GPU's neighbour-cluster counts: n11 = own-cluster kNN entries of cluster c, c2 = the
+    most frequent other cluster (ties to the smaller id, as MATLAB's mode), score = n11/(n11+n12);
+    with no other cluster the score is n11/n11 and c2 NaN (MATLAB's mode of an empty set)."""
+    P = P or {}
+    clu = np.asarray(viClu, dtype=np.int32)
+    n_clu = int(clu.max()) if nClu is None else int(nClu)
+    cnt = get_ctx(P.get("device", 0)).isolation_counts(miKnn, clu, n_clu).astype(np.int64)
+    c = np.arange(n_clu)
+    n11 = cnt[c, c + 1]
+    other = cnt[:, 1:].copy()
+    other[c, c] = 0
+    c2 = np.argmax(other, axis=1)
+    n12 = other[c, c2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        score = np.where(n12 > 0, n11 / (n11 + n12), n11 / n11.astype(np.float64))
+    nn = np.where(n12 > 0, c2 + 1, np.nan)
+    return score.astype(np.float64), nn.astype(np.float64)
+
+
+def knn_overlap_merge_(viClu, miKnn, thresh, nClu=None, P=None):
+    """knn_overlap_merge_ (matlab/irc2.m:2156-2174): clusters whose isolation score is below thresh
+    are grouped with their most frequent neighbour cluster (cell2map_); returns relabelled viClu."""
+    score, nn = isolation_score_(viClu, miKnn, nClu, P)
+    groups = [{c, int(nn[c - 1])} if score[c - 1] < thresh else {c} for c in range(1, len(score) + 1)]
+    m, _ = _merge_overlap_groups(groups)
+    clu = np.asarray(viClu).copy()
+    upd = clu > 0
+    clu[upd] = m[clu[upd] - 1]
+    return clu

@@ -20,7 +20,7 @@ EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
     "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
-    "irc_calc_drift", "irc_assign_chain",
+    "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -129,6 +129,9 @@ def load(path=None):
     lib.irc_calc_drift.restype = ctypes.c_int
     lib.irc_assign_chain.argtypes = [ctypes.c_void_p, ctypes.c_int64, _c_i64p, _c_i32p, ctypes.POINTER(Options)]
     lib.irc_assign_chain.restype = ctypes.c_int
+    lib.irc_isolation_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _c_i32p, _c_i32p,
+                                         ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Options)]
+    lib.irc_isolation_counts.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -361,6 +364,18 @@ class Context:
         lab = np.array(labels, dtype=np.int32, copy=True)
         self._check(self.lib.irc_assign_chain(self.h, lab.size, _p(nn, _c_i64p), _p(lab, _c_i32p), None))
         return lab
+
+    def isolation_counts(self, knn, clu, n_clu):
+        """Neighbour-cluster counts (n_clu, n_clu + 1) uint64: row c-1, column d = kNN entries of
+        cluster c's spikes that lie in cluster d (0: none).  knn: k x n 1-based (host)."""
+        K = np.asfortranarray(np.asarray(knn, dtype=np.int32))
+        k, n = K.shape
+        flat = np.ascontiguousarray(K.reshape(-1, order="F"))
+        cl = np.ascontiguousarray(clu, dtype=np.int32)
+        out = np.zeros((int(n_clu), int(n_clu) + 1), np.uint64)
+        self._check(self.lib.irc_isolation_counts(self.h, n, k, _p(flat, _c_i32p), _p(cl, _c_i32p), int(n_clu),
+                                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), None))
+        return out
 
     def delta_drift_dc(self, n1, miSite12, fet1, fet2, rho12, dc2):
         m = np.asfortranarray(np.asarray(miSite12, dtype=np.int32))

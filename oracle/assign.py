@@ -11,6 +11,10 @@ repository):
   cell2map_            matlab/irc2.m:6754-6803  iterated union of the overlap lists (at most 10
                                                  rounds), map to the smallest member, compact ids
   dpclus_remove_count_ matlab/irc2.m:6735-6749  clusters under min_count dropped (ids compacted)
+  isolation_score_     matlab/irc2.m:6699-6731  per cluster: own-label share of its spikes' kNN
+                                                 against the most frequent other cluster
+  knn_overlap_merge_   matlab/irc2.m:2156-2174  clusters under the isolation threshold merged
+                                                 with that neighbour through cell2map_
 
 Indices are 1-based as in MATLAB.
 """
@@ -98,3 +102,42 @@ def remove_count(clu, icl, min_count):
     c = np.asarray(clu)
     out = np.where(c > 0, vmap[np.clip(c, 0, n)], 0).astype(np.int32)
     return out, np.asarray(icl)[keep]
+
+
+def isolation_score(clu, knn, n_clu=None):
+    """isolation_score_ (matlab/irc2.m:6699-6731): for cluster c, the labels of its spikes' kNN;
+    n11 = how many are c, c2 = the most frequent other positive label (ties to the smaller, as
+    MATLAB's mode), n12 = its count; score = n11 / (n11 + n12), or n11 / n11 with c2 = NaN when no
+    other cluster occurs (MATLAB's mode of an empty set is NaN).  Returns (score, c2) float64."""
+    clu = np.asarray(clu, dtype=np.int64)
+    knn = np.asarray(knn, dtype=np.int64)
+    n_clu = int(clu.max()) if n_clu is None else int(n_clu)
+    score = np.full(n_clu, np.nan)
+    nn = np.full(n_clu, np.nan)
+    for c in range(1, n_clu + 1):
+        spikes = np.flatnonzero(clu == c)
+        lab = clu[knn[:, spikes] - 1].ravel()
+        n11 = int((lab == c).sum())
+        other = lab[(lab != c) & (lab > 0)]
+        if other.size == 0:
+            score[c - 1] = n11 / n11 if n11 else np.nan
+            continue
+        vals, cnt = np.unique(other, return_counts=True)
+        c2 = int(vals[np.argmax(cnt)])            # first maximum = smallest label among ties
+        n12 = int((lab == c2).sum())
+        score[c - 1] = n11 / (n11 + n12)
+        nn[c - 1] = c2
+    return score, nn
+
+
+def knn_overlap_merge(clu, knn, thresh, n_clu=None):
+    """knn_overlap_merge_ (matlab/irc2.m:2156-2174): clusters scoring below thresh are grouped with
+    their most frequent neighbour cluster through cell2map_; returns the relabelled clusters."""
+    score, nn = isolation_score(clu, knn, n_clu)
+    n = len(score)
+    groups = [[c, int(nn[c - 1])] if score[c - 1] < thresh else [c] for c in range(1, n + 1)]
+    m, _ = cell2map(groups)
+    clu = np.asarray(clu).copy()
+    upd = clu > 0
+    clu[upd] = m[clu[upd] - 1]
+    return clu

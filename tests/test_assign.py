@@ -63,3 +63,28 @@ def test_remove_count():
     assert list(out) == [1, 1, 0, 2, 2, 2, 0] and list(icl) == [10, 30]
     out, icl = A.remove_count(clu, np.array([10, 20, 30]), 1)
     assert list(out) == list(clu)
+
+
+def test_isolation_score_hand_case():
+    # 6 spikes, clusters 1 1 2 2 2 0, k = 2
+    clu = np.array([1, 1, 2, 2, 2, 0])
+    knn = np.array([[1, 2, 3, 4, 5, 6],
+                    [3, 6, 4, 1, 1, 5]])
+    s, nn = A.isolation_score(clu, knn)
+    # cluster 1: labels of knn[:, [1,2]] = {1, 2} and {1, 0} -> n11 = 2, other = [2] -> n12 = 1
+    # cluster 2: columns 3,4,5 -> {2,2}, {2,1}, {2,1} -> n11 = 4, other = [1,1] -> n12 = 2
+    assert s[0] == 2 / 3 and nn[0] == 2
+    assert s[1] == 4 / 6 and nn[1] == 1
+    # merging both (threshold above both scores): one cluster
+    assert list(A.knn_overlap_merge(clu, knn, 0.9)) == [1, 1, 1, 1, 1, 0]
+    assert list(A.knn_overlap_merge(clu, knn, 0.5)) == list(clu)
+
+
+def test_isolation_mode_ties_and_empty():
+    # cluster 1's neighbours: 2 and 3 once each -> MATLAB's mode picks the smaller (2)
+    clu = np.array([1, 2, 3, 4])
+    knn = np.array([[1, 2, 3, 4], [2, 2, 3, 4], [3, 2, 3, 4]])
+    s, nn = A.isolation_score(clu, knn)
+    assert nn[0] == 2 and s[0] == 1 / 2
+    # clusters 2..4 only see themselves: score 1, no neighbour (NaN)
+    assert np.all(s[1:] == 1) and np.all(np.isnan(nn[1:]))

@@ -134,6 +134,24 @@ def test_assignment_matches_oracle():
     assert host.min() >= 1
 
 
+def test_isolation_and_merge_match_oracle():
+    from oracle import assign as A
+    ds = synth.make_dataset(40000, 6, 12, 8, 6, k=16, drift_step=0.3, n_links=3, band=2, seed=53)
+    lay = irc2.site_layout(ds)
+    rho, delta, nn, knn = irc2.sort_page_(lay, {"knn": 16, "precision": "bf16"})
+    clu, _, _ = A.assign_cluster_knn(rho, delta, nn, knn, delta_cut=float(np.quantile(delta, 0.995)))
+    clu = clu.copy()
+    clu[::97] = 0                                   # some spikes outside every cluster
+    w_s, w_nn = A.isolation_score(clu, knn)
+    g_s, g_nn = irc2.isolation_score_(clu, knn)
+    np.testing.assert_array_equal(np.isnan(g_s), np.isnan(w_s))
+    np.testing.assert_array_equal(g_s[~np.isnan(w_s)], w_s[~np.isnan(w_s)])
+    np.testing.assert_array_equal(np.isnan(g_nn), np.isnan(w_nn))
+    np.testing.assert_array_equal(g_nn[~np.isnan(w_nn)], w_nn[~np.isnan(w_nn)])
+    for th in (0.5, 0.8, 0.95):
+        np.testing.assert_array_equal(irc2.knn_overlap_merge_(clu, knn, th), A.knn_overlap_merge(clu, knn, th))
+
+
 def test_full_size_site_subset_bf16():
     """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
     oracle recomputes three whole sites (their results do not depend on other sites)."""
