@@ -684,7 +684,7 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols,
                 &ranges, nullptr, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
-                                  ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
+                                  ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     HIPCHK(hipMemcpyAsync(dk_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(knn_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
@@ -884,7 +884,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
                     &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), ptr<float>(ctx->b_feat)));
         HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
-                                      nullptr, o_knn));
+                                      o_knn));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
         HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
         HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol), o_delta,
@@ -995,7 +995,7 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, &no_ranges, nullptr,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
-                                  ptr<float>(ctx->b_out_dk), nullptr, ptr<int>(ctx->b_out_knn32)));
+                                  ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
     HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
     if (I_out) HIPCHK(hipMemcpyAsync(I_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));

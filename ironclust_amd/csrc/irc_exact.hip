@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
 // ---------------------------------------------------------------------------
 // Finish the kNN pass: rho (batch mode) or d_k (site mode) and the reported ids.
 //   mode 0 (batch): rho_slot[slot] = float(1/sqrt(dk2)); dk_slot (opt) = float(sqrt(dk2));
-//                   knn64[slot*k+j] = col_ret (opt)
+//                   knn32[slot*k+j] = col_ret (opt, miKnn's int32)
 //   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret
 // A workgroup takes 64 queries: the slot-major lists are read with coalesced loads into an LDS
 // tile, then written out query-major (k contiguous entries per reported slot).
@@ -189,8 +189,7 @@ constexpr int FIN_Q = 64;
 
 __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
                                                     const double* __restrict__ dk2, float* __restrict__ rho_slot,
-                                                    float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
-                                                    int* __restrict__ knn32) {
+                                                    float* __restrict__ dk_slot, int* __restrict__ knn32) {
     __shared__ int tile[IRC_KMAX * FIN_Q];
     const int64_t i0 = (int64_t)blockIdx.x * FIN_Q;
     const int nb = nq - i0 < FIN_Q ? (int)(nq - i0) : FIN_Q;
@@ -202,7 +201,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode,
         if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
         if (dk_slot) dk_slot[slot] = (float)sqrt(d);
     }
-    if (!knn64 && !knn32) return;
+    if (!knn32) return;
     for (int e = t; e < k * FIN_Q; e += blockDim.x) {
         const int j = e / FIN_Q, c = e - j * FIN_Q;
         if (c < nb) tile[j * FIN_Q + c] = p.col_ret[knn_col[(int64_t)j * p.nq + i0 + c]];
@@ -212,8 +211,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode,
         const int c = e / k, j = e - c * k;
         const int64_t slot = p.q_out[i0 + c];
         const int v = tile[j * FIN_Q + c];
-        if (knn64) knn64[slot * k + j] = v;
-        if (knn32) knn32[slot * k + j] = v;
+        knn32[slot * k + j] = v;
     }
 }
 
@@ -530,10 +528,10 @@ hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qli
 }
 
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
-                             const double* dk2, float* rho_slot, float* dk_slot, int64_t* knn64, int* knn32) {
+                             const double* dk2, float* rho_slot, float* dk_slot, int* knn32) {
     if (nq <= 0) return hipSuccess;
     k_knn_finish<<<(unsigned)((nq + FIN_Q - 1) / FIN_Q), 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot,
-                                                                       knn64, knn32);
+                                                                       knn32);
     return hipGetLastError();
 }
 
