@@ -144,23 +144,8 @@ __device__ __forceinline__ void cex(int32_t& a, int32_t& b, bool up) {
     b = up ? hi : lo;
 }
 
-template <int N>
-__device__ __forceinline__ void bitonic_sort(int32_t* v) {
-#pragma unroll
-    for (int k = 2; k <= N; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) {
-                const int l = i ^ j;
-                if (l > i) cex<N>(v[i], v[l], (i & k) == 0);
-            }
-        }
-    }
-}
-
-// Batcher's odd-even merge sort (ascending): 191 compare-exchanges for 32 keys against the
-// bitonic network's 240
+// Batcher's odd-even merge sort (ascending): 191 compare-exchanges for 32 keys (a bitonic
+// sorting network needs 240)
 template <int N>
 __device__ __forceinline__ void oddeven_sort(int32_t* v) {
 #pragma unroll
