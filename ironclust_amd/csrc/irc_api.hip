@@ -379,7 +379,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     if (kprime < p.k) kprime = p.k;
     kprime = kprime <= 32 ? 32 : (kprime <= 40 ? 40 : (kprime <= 48 ? 48 : 64));
     const int large_ncand = ctx->large_ncand;
-    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * ((size_t)nq + 1)));
+    // [0, nq) exact-kernel list, [nq] its count, [nq + 1, 2 nq + 1) the first re-rank pass's overflows
+    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * (2 * (size_t)nq + 1)));
     CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
     int* fb = ptr<int>(ctx->b_scr[4]);
     int* nfb = fb + nq;
@@ -492,7 +493,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
     int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
-    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2));
+    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, seg_ctr + 20));
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));

@@ -62,9 +62,12 @@ hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, i
 // per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,
                                const int* seg_beg);
-// FP64 re-rank of the certified lists into the k-lists (knn_col, dk2); queries whose
-// ambiguous prefix does not fit the re-rank window join the fallback list.
-hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2);
+// FP64 re-rank of the certified lists into the k-lists (knn_col, dk2) in two passes: a short
+// window over every query, whose overflows (spill[0 .. *n_spill), n_q ints of room; the counter
+// is zeroed by the launch) take a second pass with the long window; queries whose ambiguous
+// prefix does not fit that either join the fallback list.
+hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2, int* spill,
+                                int* n_spill);
 hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 // largest |x|^2 of every (block, part) column run
 hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,

@@ -205,8 +205,6 @@ __device__ __forceinline__ float key_dist(int32_t key, uint32_t hmask) {
 constexpr int32_t KEY_EMPTY = 0x7fffffff;
 constexpr int32_t KEY_NONE = (int32_t)0x80000000;   // threshold of lanes without a query
 
-constexpr int WIN = 16;  // re-rank window: longest run of mutually ambiguous neighbours
-
 template <int PREC, int KB, int KPB, bool HIST>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
                                                    const int* __restrict__ seg_beg) {
@@ -626,28 +624,24 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 // pieces of a row, straight into LDS (global_load_lds), so a load instruction touches a few
 // rows instead of 64 -- and each lane then reads its own row back and runs the canonical FP64
 // chain against its query row, held in registers.  The next slot's gather is in flight while
-// the chain runs.  Each lane keeps a window of WIN exact distances in (d^2, id) order and emits
+// the chain runs.  Each lane keeps a window of W exact distances in (d^2, id) order and emits
 // its head once the head is below the next slot's lower bound d_j - eps (no later slot can
-// precede it).  A window overflow (more than WIN mutually ambiguous candidates) sends the
-// query to the exact kernel.
-template <int NV>
-__global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int* __restrict__ cand_n,
-                                              const int* __restrict__ cand_col, const float* __restrict__ cand_d,
-                                              const double* __restrict__ cand_eps, int* __restrict__ knn_col,
-                                              double* __restrict__ dk2,
-                                              int* __restrict__ fallback, int* __restrict__ n_fallback) {
-    __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
-    __shared__ int skey[64];            // their spike ids
-    __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
-    extern __shared__ int obuf[];       // [k][64] emitted columns when k <= 32 (else written directly)
+// precede it).  A window overflow (more than W mutually ambiguous candidates) sends the query
+// on: the first pass keeps a short window (its predicated insertion and shifts are most of the
+// VALU work) and hands overflows to a second pass over that list with the long window; what
+// overflows there goes to the exact kernel.
+constexpr int WIN1 = 8;    // first pass: every certified query
+constexpr int WIN2 = 16;   // second pass: the first pass's overflows
+
+// one group of up to 64 queries (lane l = query q, valid when qv); overflows are appended to spill
+template <int NV, int W>
+__device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int64_t q, bool qv,
+                                             const int* __restrict__ cand_n, const int* __restrict__ cand_col,
+                                             const float* __restrict__ cand_d, const double* __restrict__ cand_eps,
+                                             int* __restrict__ knn_col, double* __restrict__ dk2,
+                                             int* __restrict__ spill, int* __restrict__ n_spill, float4* stage,
+                                             int* skey, float* sdn, int* obuf) {
     const int l = threadIdx.x;
-    // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
-    // of the query groups, so the candidate rows its groups share stay in its L2
-    const unsigned G = gridDim.x, x = blockIdx.x & 7, per = (G + 7) >> 3, lo8 = G & 7;
-    const unsigned grp = (lo8 == 0 || x < lo8) ? x * per + (blockIdx.x >> 3)
-                                               : lo8 * per + (x - lo8) * (per - 1) + (blockIdx.x >> 3);
-    const int64_t q = (int64_t)grp * 64 + l;
-    const bool qv = q < n_q;
     const int rn = qv ? cand_n[q] : -1;
     const int r = rn < 0 ? 0 : (rn & 0xffff);
     int rmax = r;
@@ -686,8 +680,8 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
     const int k = p.k;
     const int kk = rn >> 16;
     const double eps = qv && rn >= 0 ? cand_eps[q] : 0.0;
-    double wt[WIN];
-    int wk[WIN], wc[WIN];
+    double wt[W];
+    int wk[W], wc[W];
     int wn = 0, out = 0;
     double last_t = 0.0;
     int last_c = -1;
@@ -720,12 +714,12 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
                 d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
             }
             const double lb = j + 1 < r ? (double)dn - eps : __builtin_inf();
-            if (wn == WIN) {
+            if (wn == W) {
                 ok = false;
             } else {
                 bool placed = false;
 #pragma unroll
-                for (int i = WIN - 1; i >= 0; --i) {
+                for (int i = W - 1; i >= 0; --i) {
                     if (i <= wn && !placed) {
                         if (i > 0 && irc_less(t, kc, wt[i - 1], wk[i - 1])) {
                             wt[i] = wt[i - 1]; wk[i] = wk[i - 1]; wc[i] = wc[i - 1];
@@ -743,7 +737,7 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
                     last_c = wc[0];
                     ++out;
 #pragma unroll
-                    for (int i = 0; i < WIN - 1; ++i) {
+                    for (int i = 0; i < W - 1; ++i) {
                         if (i + 1 < wn) { wt[i] = wt[i + 1]; wk[i] = wk[i + 1]; wc[i] = wc[i + 1]; }
                     }
                     --wn;
@@ -761,13 +755,54 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
         }
         dk2[q] = last_t;
     } else if (qv && rn >= 0) {
-        const int f = atomicAdd(n_fallback, 1);
-        fallback[f] = (int)q;
+        const int f = atomicAdd(n_spill, 1);
+        spill[f] = (int)q;
     }
-    // the buffered lists leave slot by slot: 64 consecutive queries per store
+    // the buffered lists leave slot by slot: 64 queries per store
     if (k <= 32) {
         for (int j = 0; j < k; ++j)
             if (done) knn_col[(int64_t)j * n_q + q] = obuf[j * 64 + l];
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int* __restrict__ cand_n,
+                                              const int* __restrict__ cand_col, const float* __restrict__ cand_d,
+                                              const double* __restrict__ cand_eps, int* __restrict__ knn_col,
+                                              double* __restrict__ dk2, int* __restrict__ spill,
+                                              int* __restrict__ n_spill) {
+    __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
+    __shared__ int skey[64];            // their spike ids
+    __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
+    extern __shared__ int obuf[];       // [k][64] emitted columns when k <= 32 (else written directly)
+    // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
+    // of the query groups, so the candidate rows its groups share stay in its L2
+    const unsigned G = gridDim.x, x = blockIdx.x & 7, per = (G + 7) >> 3, lo8 = G & 7;
+    const unsigned grp = (lo8 == 0 || x < lo8) ? x * per + (blockIdx.x >> 3)
+                                               : lo8 * per + (x - lo8) * (per - 1) + (blockIdx.x >> 3);
+    const int64_t q = (int64_t)grp * 64 + threadIdx.x;
+    rerank_group<NV, WIN1>(p, n_q, q, q < n_q, cand_n, cand_col, cand_d, cand_eps, knn_col, dk2, spill, n_spill,
+                           stage, skey, sdn, obuf);
+}
+
+// second pass: the queries of list[0 .. *n_list), 64 per group, groups strided over the grid
+template <int NV>
+__global__ __launch_bounds__(64) void k_rerank_list(DevPlan p, int64_t n_q, const int* __restrict__ list,
+                                                   const int* __restrict__ n_list, const int* __restrict__ cand_n,
+                                                   const int* __restrict__ cand_col, const float* __restrict__ cand_d,
+                                                   const double* __restrict__ cand_eps, int* __restrict__ knn_col,
+                                                   double* __restrict__ dk2, int* __restrict__ fallback,
+                                                   int* __restrict__ n_fallback) {
+    __shared__ float4 stage[64 * NV];
+    __shared__ int skey[64];
+    __shared__ float sdn[64];
+    extern __shared__ int obuf[];
+    const int64_t n = *n_list;
+    for (int64_t g = blockIdx.x; g * 64 < n; g += gridDim.x) {
+        const int64_t i = g * 64 + threadIdx.x;
+        const bool qv = i < n;
+        rerank_group<NV, WIN2>(p, n_q, qv ? (int64_t)list[i] : 0, qv, cand_n, cand_col, cand_d, cand_eps, knn_col,
+                               dk2, fallback, n_fallback, stage, skey, sdn, obuf);
     }
 }
 
@@ -897,14 +932,19 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArg
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2) {
+hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2, int* spill,
+                                int* n_spill) {
     if (a.n_q <= 0) return hipSuccess;
+    k_zero_ints<<<1, 64, 0, st>>>(n_spill, 1);
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
+    const unsigned grid2 = grid < 1024 ? grid : 1024;
     const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
         k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, \
-                                          dk2, a.fallback, a.n_fallback);                                  \
+                                          dk2, spill, n_spill);                                            \
+        k_rerank_list<NV><<<grid2, 64, obytes, st>>>(a.p, a.n_q, spill, n_spill, a.cand_n, a.cand_col, a.cand_d, \
+                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback);       \
         break;
     switch (a.p.DP / 4) {
         IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)
