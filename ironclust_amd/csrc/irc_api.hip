@@ -452,8 +452,6 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     const size_t pcols = (size_t)irc::screen_cols_padded(n_cols);   // tiled rows, whole 32-column groups
     CHK(ensure(ctx, ctx->b_scr[0], eb * pcols * KP));
     CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
-    CHK(ensure(ctx, ctx->b_scr[6], eb * pcols * KP));
-    CHK(ensure(ctx, ctx->b_scr[7], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
     CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
     // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
     const int slots = (kprime > 32 && any_large) ? 64 : kprime;
@@ -462,7 +460,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->b_cand_eps, sizeof(double) * (size_t)nq));
     HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
-                                   ctx->b_scr[6].p, ctx->b_scr[7].p, ptr<float>(ctx->b_scr[2]), feat_out, p.DP));
+                                   ptr<float>(ctx->b_scr[2]), feat_out, p.DP));
     float* run_nmax = ptr<float>(ctx->b_scr[2]) + n_cols;
     HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->b_scr[2]),
                                 run_nmax));
@@ -471,8 +469,6 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.run_nmax = run_nmax;
     a.hi = ctx->b_scr[0].p;
     a.lo = ctx->b_scr[1].p;
-    a.qhi = ctx->b_scr[6].p;
-    a.qlo = ctx->b_scr[7].p;
     a.norm = ptr<float>(ctx->b_scr[2]);
     a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
     a.n_items = n_items_cap;

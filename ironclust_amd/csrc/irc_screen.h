@@ -29,8 +29,6 @@ struct ScreenArgs {
     DevPlan p;
     const void* hi;          // tiled rows: bf16 bits (bf16 form) or fp32 values (fp32 form)
     const void* lo;          // tiled rows: bf16 low parts (bf16 form)
-    const void* qhi;         // tiled query-form rows [-2x, |x|^2, 1, 0...] (hi parts or fp32)
-    const void* qlo;         // tiled query-form low parts (bf16 form)
     const float* norm;       // [n_cols] fp32 |x|^2
     const float* run_nmax;   // [n_blocks * 2] largest |x|^2 of each (block, part) run
     const ScreenItem* items;
@@ -56,7 +54,7 @@ int screen_kp(int D);                                  // padded operand width (
 double screen_eta(int prec, int D, int KP);            // rigorous error factor
 // feat (optional): also write the [n_cols][DP] zero-padded FP32 rows (F must have stride D)
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, void* qhi, void* qlo, float* norm, float* feat = nullptr,
+                              void* hi, void* lo, float* norm, float* feat = nullptr,
                               int DP = 0);
 // seg_ctr: 8 device counters (zeroed by the launch); seg_beg: 9 device offsets of the
 // per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).

@@ -45,14 +45,14 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 
 // A workgroup packs 32 columns: their rows are staged in LDS with coalesced loads, one thread
 // per column forms |x|^2, then one thread per (column, 8-dimension chunk) writes coalesced
-// 16-byte pieces of the candidate-form rows [x, 1, |x|^2, 0...] and the query-form rows
-// [-2x, |x|^2, 1, 0...] (bf16 hi/lo split, or fp32).
+// 16-byte pieces of the candidate-form rows [x, 1, |x|^2, 0...] (bf16 hi/lo split, or fp32).
+// The query-form rows [-2x, |x|^2, 1, 0...] are not stored: the screen derives them from the
+// candidate-form rows and |x|^2 (qform_bf16 / qform_f32).
 constexpr int PACK_COLS = 32;
 
 __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
-                              void* __restrict__ hi, void* __restrict__ lo, void* __restrict__ qhi,
-                              void* __restrict__ qlo, float* __restrict__ norm, float* __restrict__ feat,
-                              int DP) {
+                              void* __restrict__ hi, void* __restrict__ lo, float* __restrict__ norm,
+                              float* __restrict__ feat, int DP) {
     __shared__ float srow[PACK_COLS * IRC_DMAX];
     __shared__ float snorm[PACK_COLS];
     const int nch = KP / 8;
@@ -90,48 +90,64 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
     const int64_t col = col0 + c;
     const float* x = srow + c * IRC_DMAX;
     const float n = snorm[c];
-    float cv[8], qv[8];
+    float cv[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         const int d = 8 * j + t;
-        const float xv = d < D ? x[d] : 0.0f;
-        cv[t] = d < D ? xv : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
-        qv[t] = d < D ? xv : (d == D ? n : (d == D + 1 ? 1.0f : 0.0f));
+        cv[t] = d < D ? x[d] : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
     }
     if (prec == SCREEN_FP32) {
         // fp32 rows are KP / 4 pieces: this thread's 8 values are pieces 2 j and 2 j + 1
         float4* h4 = reinterpret_cast<float4*>(hi);
-        float4* q4 = reinterpret_cast<float4*>(qhi);
         const int np = KP / 4;
         h4[screen_piece(col, 2 * j, np)] = make_float4(cv[0], cv[1], cv[2], cv[3]);
         h4[screen_piece(col, 2 * j + 1, np)] = make_float4(cv[4], cv[5], cv[6], cv[7]);
-        float s[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) s[t] = 8 * j + t < D ? -2.0f * qv[t] : qv[t];
-        q4[screen_piece(col, 2 * j, np)] = make_float4(s[0], s[1], s[2], s[3]);
-        q4[screen_piece(col, 2 * j + 1, np)] = make_float4(s[4], s[5], s[6], s[7]);
     } else {
-        uint32_t ph[4], pl[4], qh[4], ql[4];
+        uint32_t ph[4], pl[4];
 #pragma unroll
         for (int t = 0; t < 8; t += 2) {
             uint16_t h0 = f2bf_rne(cv[t]), h1 = f2bf_rne(cv[t + 1]);
             uint16_t l0 = f2bf_rne(cv[t] - bf2f(h0)), l1 = f2bf_rne(cv[t + 1] - bf2f(h1));
             ph[t / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
             pl[t / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-            // -2 scales the hi and lo parts of the query value exactly
-            const float w0 = 8 * j + t < D ? -2.0f : 1.0f, w1 = 8 * j + t + 1 < D ? -2.0f : 1.0f;
-            uint16_t a0 = f2bf_rne(qv[t]), a1 = f2bf_rne(qv[t + 1]);
-            uint16_t b0 = f2bf_rne(qv[t] - bf2f(a0)), b1 = f2bf_rne(qv[t + 1] - bf2f(a1));
-            qh[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(a0)) | ((uint32_t)f2bf_rne(w1 * bf2f(a1)) << 16);
-            ql[t / 2] = (uint32_t)f2bf_rne(w0 * bf2f(b0)) | ((uint32_t)f2bf_rne(w1 * bf2f(b1)) << 16);
         }
         // bf16 rows are KP / 8 pieces: this thread's 8 values are piece j
         const int64_t o = screen_piece(col, j, KP / 8);
         reinterpret_cast<uint4*>(hi)[o] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
         reinterpret_cast<uint4*>(lo)[o] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
-        reinterpret_cast<uint4*>(qhi)[o] = make_uint4(qh[0], qh[1], qh[2], qh[3]);
-        reinterpret_cast<uint4*>(qlo)[o] = make_uint4(ql[0], ql[1], ql[2], ql[3]);
     }
+}
+
+// Query-form piece [-2x, |x|^2, 1, 0...] of dimensions d0 .. d0+7 from the candidate-form piece
+// [x, 1, |x|^2, 0...] of the same column: below D every part is scaled by -2 (exact in bf16),
+// at D and D+1 the parts of |x|^2 and of 1 are put in (at_D, at_D1).
+__device__ __forceinline__ uint4 qform_bf16(uint4 v, bool valid, int d0, int D, uint16_t at_D, uint16_t at_D1) {
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uint32_t o = 0;
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const int d = d0 + 2 * t + hh;
+            const uint16_t b = (uint16_t)(w[t] >> (16 * hh));
+            const uint16_t r = d < D ? f2bf_rne(-2.0f * bf2f(b)) : (d == D ? at_D : (d == D + 1 ? at_D1 : 0));
+            o |= (uint32_t)r << (16 * hh);
+        }
+        w[t] = valid ? o : 0u;
+    }
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// the same for an fp32 piece (dimensions d0 .. d0+3)
+__device__ __forceinline__ float4 qform_f32(float4 v, bool valid, int d0, int D, float nrm) {
+    float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int d = d0 + t;
+        const float r = d < D ? -2.0f * c[t] : (d == D ? nrm : (d == D + 1 ? 1.0f : 0.0f));
+        c[t] = valid ? r : 0.0f;
+    }
+    return make_float4(c[0], c[1], c[2], c[3]);
 }
 
 // keys are signed: a negative screened distance (a candidate within the error bound of the
@@ -282,7 +298,8 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         __syncthreads();
         const int ncand = item.ncand;
 
-        // ---- query operands (B) for the two 32-query column tiles (query-form rows)
+        // ---- query operands (B) for the two 32-query column tiles: query-form rows derived from
+        // the candidate-form rows of the query columns
         bf16x8 bh[2][KB], bl[2][KB];
         float4 bq[2][2 * KB];
 #pragma unroll
@@ -290,21 +307,25 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const int qj = 32 * T + m;
             const bool qv = qj < item.nq;
             const int qc = p.q_col[item.q0 + (qv ? qj : 0)];
+            const float qn = a.norm[qc];
             if constexpr (PREC == SCREEN_BF16) {
-                const float4* rh = reinterpret_cast<const float4*>(a.qhi);
-                const float4* rl = reinterpret_cast<const float4*>(a.qlo);
+                const uint4* rh = reinterpret_cast<const uint4*>(a.hi);
+                const uint4* rl = reinterpret_cast<const uint4*>(a.lo);
+                const uint16_t nh = f2bf_rne(qn), nl = f2bf_rne(qn - bf2f(nh));
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                    const int64_t o = screen_piece(qc, 2 * kb + h, 2 * KB);
-                    bh[T][kb] = __builtin_bit_cast(bf16x8, qv ? rh[o] : z);
-                    bl[T][kb] = __builtin_bit_cast(bf16x8, qv ? rl[o] : z);
+                    const int pc = 2 * kb + h;
+                    const int64_t o = screen_piece(qc, pc, 2 * KB);
+                    bh[T][kb] = __builtin_bit_cast(bf16x8, qform_bf16(rh[o], qv, 8 * pc, p.D, nh, f2bf_rne(1.0f)));
+                    bl[T][kb] = __builtin_bit_cast(bf16x8, qform_bf16(rl[o], qv, 8 * pc, p.D, nl, 0));
                 }
             } else {
-                const float4* r = reinterpret_cast<const float4*>(a.qhi);
+                const float4* r = reinterpret_cast<const float4*>(a.hi);
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g)
-                    bq[T][g] = qv ? r[screen_piece(qc, 2 * g + h, 4 * KB)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int g = 0; g < 2 * KB; ++g) {
+                    const int pc = 2 * g + h;
+                    bq[T][g] = qform_f32(r[screen_piece(qc, pc, 4 * KB)], qv, 4 * pc, p.D, qn);
+                }
             }
         }
         // after the lane swap, lane l owns query l of the item
@@ -900,12 +921,12 @@ double screen_eta(int prec, int D, int KP) {
 }
 
 hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, void* qhi, void* qlo, float* norm, float* feat, int DP) {
+                              void* hi, void* lo, float* norm, float* feat, int DP) {
     if (feat && FS != D) return hipErrorInvalidValue;
     if (n_cols <= 0) return hipSuccess;
     const int64_t blocks = (n_cols + PACK_COLS - 1) / PACK_COLS;
-    k_screen_pack<<<(unsigned)blocks, PACK_COLS * (KP / 8), 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, qhi, qlo,
-                                                                     norm, feat, DP);
+    k_screen_pack<<<(unsigned)blocks, PACK_COLS * (KP / 8), 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, norm,
+                                                                     feat, DP);
     return hipGetLastError();
 }
 
