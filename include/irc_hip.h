@@ -197,6 +197,8 @@ typedef struct {
     double ms_rerank;         /* the FP64 re-rank of the certified lists (0 on the exact path) */
     int64_t n_rerank_overflow;/* of n_screen_fallback: queries whose ambiguous run overflowed the re-rank window */
     double ms_graph;          /* device time of the replayed graph (IRC_FLAG_GRAPH replays; else 0) */
+    int64_t n_rerank_second;  /* queries the re-rank's long-window second pass redid (their ambiguous
+                                 run overflowed the first pass's short window) */
 } irc_stats;
 
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);

@@ -356,6 +356,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     ctx->stats.n_screen_fallback = 0;
     ctx->stats.ms_rerank = 0.0;
     ctx->stats.n_rerank_overflow = 0;
+    ctx->stats.n_rerank_second = 0;
     ctx->dstat_screen = false;
     CHK(ensure(ctx, ctx->b_dstat, sizeof(int64_t) * 8));
     HIPCHK(irc::launch_zero_i64(st, ptr<int64_t>(ctx->b_dstat), 8));
@@ -490,6 +491,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
     HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, seg_ctr + 20));
+    HIPCHK(irc::launch_stat_copy(st, seg_ctr + 20, nullptr, dstat + 3));
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));
@@ -505,6 +507,7 @@ irc_status collect_stats(irc_ctx* ctx, bool events) {
     if (ctx->dstat_screen) {
         ctx->stats.n_screen_fallback = d[1];
         ctx->stats.n_rerank_overflow = d[1] - d[0];
+        ctx->stats.n_rerank_second = d[3];
         if (d[2] > 0) ctx->stats.n_pairs = d[2];
         if (events) {
             ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);

@@ -66,7 +66,7 @@ class Stats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_knn", ctypes.c_double), ("ms_delta", ctypes.c_double),
                 ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32),
                 ("ms_rerank", ctypes.c_double), ("n_rerank_overflow", ctypes.c_int64),
-                ("ms_graph", ctypes.c_double)]
+                ("ms_graph", ctypes.c_double), ("n_rerank_second", ctypes.c_int64)]
 
 
 class IrcError(RuntimeError):

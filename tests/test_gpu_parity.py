@@ -58,6 +58,23 @@ def test_batch_fewer_candidates_than_k(prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_rerank_second_pass(prec, ctx):
+    # integer features in 6 dimensions: runs of tied distances longer than the first re-rank
+    # pass's window send thousands of queries through the second (long-window) pass, and the
+    # longest runs on to the exact kernel; every route must give the oracle's lists
+    ds = int_dataset(N=20000, S=2, C=4, D=6, k=16, seed=62, lo=-3, hi=4)
+    _check_batch(ds, prec, "rerank-second")
+    lay = irc2.site_layout({kk: (__import__("torch").as_tensor(v) if kk in ("F1", "F2", "site1", "site2")
+                                 and v is not None else v) for kk, v in ds.items()})
+    host = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in lay.items()}
+    ctx.rho_delta_batch(host, ds["k"], opts=libirc.make_options(libirc.PREC_BF16_SCREEN if prec == "bf16"
+                                                                  else libirc.PREC_FP32))
+    st = ctx.stats()
+    assert st["n_rerank_second"] > 100, st
+    assert st["n_rerank_overflow"] <= st["n_rerank_second"] <= st["n_queries"], st
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_batch_large_blocks_screened(prec, ctx):
     # blocks of 15k-22k candidates: stream positions need 15 bits of the screen key, so the
     # truncated keys are coarser; the screen must still certify almost every query
