@@ -46,7 +46,7 @@ struct irc_ctx {
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
     DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
-    DevBuf b_scr[8];
+    DevBuf b_scr[6];   // screen: [0] hi, [1] lo, [2] norms + run maxima, [3] items, [4] fallback/spill lists, [5] counters
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
     DevBuf b_plan;                                         // device-side work planning
     DevBuf b_drift;                                        // drift linkage scratch
