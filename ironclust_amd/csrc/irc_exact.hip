@@ -304,7 +304,10 @@ __global__ __launch_bounds__(64) void k_delta_from_knn(DevPlan p, int nq, const 
 // ---------------------------------------------------------------------------
 // Exact nearest denser neighbour: one wavefront per query, lanes over candidates.  Only denser
 // candidates (rho_b > rho_a, usually a few percent for the queries that reach this kernel) are
-// scored, so each lane loads just those rows; the query row is held in registers.
+// scored: each 64-candidate chunk is filtered on rho and its denser columns are appended to an
+// LDS queue (ballot + prefix count), and the FP64 chain runs once per 64 queued columns, not once
+// per chunk that holds any.  The (d2, id) minimum does not depend on the scoring order.  The
+// query row is held in registers.
 //   mode 0 (batch): delta = float(sqrt(d2)*rho_a) with the sentinel rule; nneigh int64
 //   mode 1 (site) : dmin = float(sqrt(d2)) or +Inf; nneigh32 = col_ret (own when none)
 // The query list is qlist[0 .. *nq_dev) when nq_dev is set, else qlist[0..nq) / identity.
@@ -315,7 +318,9 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
                                                     float* __restrict__ delta_slot,
                                                     int64_t* __restrict__ nneigh64,
                                                     int* __restrict__ nneigh32) {
+    __shared__ int qbuf[128];   // queued denser columns (fewer than 64 carried + one chunk)
     const int lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
     const int n = nq_dev ? *nq_dev : nq;
     for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
         const int q = qlist ? qlist[qi] : qi;
@@ -331,32 +336,64 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
         }
         double bt = __builtin_inf();
         int bk = 0x7fffffff, bc = -1;
+        // lane l scores queued column l (l < cnt)
+        auto score = [&](int cnt) {
+            if (lane < cnt) {
+                const int col = qbuf[lane];
+                const float4* x4 = reinterpret_cast<const float4*>(p.feat + (int64_t)col * p.DP);
+                float4 b[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) b[i] = x4[i];
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    double d;
+                    d = (double)qr[i].x - (double)b[i].x; t = fma(d, d, t);
+                    d = (double)qr[i].y - (double)b[i].y; t = fma(d, d, t);
+                    d = (double)qr[i].z - (double)b[i].z; t = fma(d, d, t);
+                    d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
+                }
+                const int key = p.col_key[col];
+                if (bc < 0 || irc_less(t, key, bt, bk)) { bt = t; bk = key; bc = col; }
+            }
+        };
+        int nbuf = 0;   // wave-uniform queue length, < 64 between chunks
         for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
             const int b2 = s * p.C + p.link_idx[li];
             for (int part = 0; part < 2; ++part) {
                 const int beg = p.run_beg[2 * b2 + part];
                 const int len = p.run_len[2 * b2 + part];
-                for (int j = lane; j < len; j += 64) {
-                    const int col = beg + j;
-                    if (!(rho_col[col] > ra)) continue;
-                    const float4* x4 = reinterpret_cast<const float4*>(p.feat + (int64_t)col * p.DP);
-                    float4 b[NV];
+                // four chunks' rho loads in flight at once, then filtered chunk by chunk
+                for (int j4 = 0; j4 < len; j4 += 256) {
+                    float rj[4];
 #pragma unroll
-                    for (int i = 0; i < NV; ++i) b[i] = x4[i];
-                    double t = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NV; ++i) {
-                        double d;
-                        d = (double)qr[i].x - (double)b[i].x; t = fma(d, d, t);
-                        d = (double)qr[i].y - (double)b[i].y; t = fma(d, d, t);
-                        d = (double)qr[i].z - (double)b[i].z; t = fma(d, d, t);
-                        d = (double)qr[i].w - (double)b[i].w; t = fma(d, d, t);
+                    for (int u = 0; u < 4; ++u) {
+                        const int j = j4 + 64 * u + lane;
+                        rj[u] = j < len ? rho_col[beg + j] : 0.0f;
                     }
-                    const int key = p.col_key[col];
-                    if (bc < 0 || irc_less(t, key, bt, bk)) { bt = t; bk = key; bc = col; }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int j = j4 + 64 * u + lane;
+                        const bool den = j < len && rj[u] > ra;
+                        const uint64_t m = __ballot(den);
+                        if (den) qbuf[nbuf + __popcll(m & below)] = beg + j;
+                        nbuf += __popcll(m);
+                        if (nbuf >= 64) {
+                            __syncthreads();
+                            score(64);
+                            const int rest = nbuf - 64;
+                            const int mv = lane < rest ? qbuf[64 + lane] : 0;
+                            __syncthreads();
+                            if (lane < rest) qbuf[lane] = mv;
+                            nbuf = rest;
+                        }
+                    }
                 }
             }
         }
+        __syncthreads();
+        score(nbuf);
+        __syncthreads();   // the queue is reused by the next query
         int wl = lane;
         if (bc < 0) { bt = __builtin_inf(); bk = 0x7fffffff; }
         wave_argmin(bt, bk, bc, wl);
