@@ -52,7 +52,7 @@ def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                                   "(the path would shard by site, see DESIGN.md)"}))
+                                   "(the path would shard by site)"}))
         sys.exit(2)
     import numpy as np
     import torch
