@@ -80,7 +80,7 @@ def main():
     out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
                delta=torch.empty(N, dtype=torch.float32, device="cuda"),
                nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
-               knn=torch.empty((N, k), dtype=torch.int32, device="cuda"))
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
     ctx = libirc.Context(0)
     forms = {}
     results = {}

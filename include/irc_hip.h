@@ -12,8 +12,9 @@
  *                                            rho_paged_site_ 3635, delta_paged_site_ 3684), all sites and
  *                                            drift chunks in one call
  *   irc_knn_ab          cuda_knn_           matlab/irc2.m:4202-4238, irc.m:24365-24401
- *                                            (kernels cuda_knn_index.cu:27-111, cuda_knn.cu, cuda_knn_full.cu,
- *                                            cuda_knn_ib.cu)
+ *                                            (kernels cuda_knn_index.cu:27-111, cuda_knn.cu)
+ *   irc_knn_ab_full     cuda_knn_full       matlab/cuda_knn_full.cu:27-100 (all k distances)
+ *   irc_knn_ab_ib       cuda_knn_ib         matlab/cuda_knn_ib.cu:27-97 (indirect candidate list)
  *   irc_delta_knn_ab    cuda_delta_knn_     irc.m:24519-24553        (kernel cuda_delta_knn.cu:19-88)
  *   irc_rho_drift_dc    cuda_rho_drift_     irc.m:24612-24637        (kernel irc_cuda_rho_drift.cu:28-108)
  *   irc_delta_drift_dc  cuda_delta_drift_   irc.m:24641-24666        (kernel irc_cuda_delta_drift.cu:28-132)
@@ -65,12 +66,17 @@ typedef enum {
 } irc_mem;
 
 /* irc_options.flags */
-#define IRC_FLAG_HIST 2   /* run the screen's histogram pre-pass (a first pass that sets each query's
-                             starting threshold; off by default) */
 #define IRC_FLAG_GRAPH 4  /* irc_rho_delta_batch: capture the device phase (kNN, rho, delta) in a HIP
                              graph and replay it on later calls with the same descriptor: same
                              pointers, sizes, options and host descriptor contents, and unchanged
-                             col_spike contents (only F may change between calls) */
+                             col_spike contents (only F may change between calls).  A call that grows
+                             a context buffer invalidates the graph; the next call captures again. */
+#define IRC_FLAG_AUTOTUNE 8 /* irc_rho_delta_batch: use the context's tuned options for this workload
+                             shape (irc_autotune runs first when there are none yet) */
+#define IRC_FLAG_SITE_LAUNCHES 16 /* irc_rho_delta_batch: launch the screen once per site over that
+                             site's work items instead of once persistently over all of them (the
+                             reference's per-site launch pattern, matlab/irc2.m:3537-3561; for
+                             comparison; not with IRC_FLAG_GRAPH) */
 
 typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
@@ -168,20 +174,41 @@ typedef struct {
  *   rho     float32(1/sqrt(d2_k))                    (vrRho before rho/max(rho), irc2.m:3669)
  *   delta   float32(sqrt(d2_min) * rho)              (vrDelta, irc2.m:3720-3725)
  *   nneigh  1-based spike id of the nearest denser spike, or self
- *   knn     k x n_spikes int32 (miKnn's type), 1-based spike ids, nearest first (may be NULL)
- *           (_knn_<site>.irc)
+ *   knn     k x n_spikes int64, 1-based spike ids, nearest first (may be NULL): miKnn_in after
+ *           int64(viSpk_out(miKnn_in)) (irc2.m:3670), the type _knn_<site>.irc holds (irc2.m:3395)
  *   dk      sqrt(d2_k) (may be NULL)
  */
 typedef struct {
     float* rho;
     float* delta;
     int64_t* nneigh;
-    int32_t* knn;
+    int64_t* knn;
     float* dk;
 } irc_outputs;
 
 irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* batch, const irc_options* opt,
                                irc_outputs* out);
+
+/* Autotuner of irc_rho_delta_batch (no reference counterpart: the reference fixes its launch shapes,
+ * matlab/irc2.m:3755-3757, 3775).  Times the call on the caller's batch and outputs with each
+ * candidate setting of the over-fetch depth (k_over), the screen's persistent grid (waves_per_cu), the
+ * smallest query group screened per wavefront (tile, the queue granularity) and the deep-list
+ * threshold (large_ncand), starting from *base (its precision and mem are kept), and writes the
+ * fastest whose outputs equal base's bit for bit to *best.  The choice is also kept in the context for
+ * IRC_FLAG_AUTOTUNE calls of the same workload shape.  The outputs hold base's results on return.
+ * rep (may be NULL) receives the trials. */
+typedef struct {
+    int32_t n_trials;
+    int32_t best_trial;        /* index into the trial arrays */
+    double ms_base;            /* device time of base's setting */
+    double ms_best;
+    int32_t k_over[16], waves_per_cu[16], tile[16], large_ncand[16];
+    double ms[16];
+    int32_t identical[16];     /* the trial reproduced base's outputs */
+} irc_tune_report;
+
+irc_status irc_autotune(irc_ctx* ctx, const irc_batch* batch, const irc_options* base, irc_outputs* out,
+                        irc_options* best, irc_tune_report* rep);
 
 /* Statistics of the last irc_rho_delta_batch / irc_knn_drift call (for benchmarks and tests). */
 typedef struct {
@@ -215,6 +242,18 @@ irc_status irc_selftest_mfma(irc_ctx* ctx, float* out);
  */
 irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A,
                       int32_t k, float* K_out, int32_t* I_out);
+
+/* irc_knn_ab_full: the cuda_knn_full form (cuda_knn_full.cu:27-100; the kernel has no caller in the
+ * reference's MATLAB): K_out[k x nA] = sqrt(d2) of every one of the k nearest (nearest first, not only
+ * the k-th), I_out[k x nA] = 1-based indices into B.  Exact, ties by lower B index. */
+irc_status irc_knn_ab_full(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A,
+                           int32_t k, float* K_out, int32_t* I_out);
+
+/* irc_knn_ab_ib: the cuda_knn_ib form (cuda_knn_ib.cu:27-97): the candidates are the columns
+ * IB[0..nIB) (1-based, cuda_knn_ib.cu:59) of B.  K_out[nA] = sqrt(d2_k); I_out[k x nA] (may be
+ * NULL) = 1-based columns of B (not positions in IB).  Exact, ties by lower B column. */
+irc_status irc_knn_ab_ib(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nIB, const int32_t* IB,
+                         int64_t nA, const float* A, int32_t k, float* K_out, int32_t* I_out);
 
 /* irc_delta_knn_ab: min distance from each A to B with R_B > R_A (cuda_delta_knn.cu:19-88);
  * D_out = sqrt(d2_min) (+Inf when none), N_out 1-based index into B (0 when none). */
@@ -272,10 +311,10 @@ irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int3
 
 /* Neighbour-cluster counts behind isolation_score_ (matlab/irc2.m:6699-6731), used by
  * knn_overlap_merge_ (matlab/irc2.m:2156-2174): counts[(c-1)*(n_clu+1) + d] = how many kNN entries
- * of the spikes of cluster c (1..n_clu) lie in cluster d (0: in no cluster).  knn: k x n int32
- * 1-based spike ids (the batch call's output); clu[n]: cluster per spike (<= 0: none).
+ * of the spikes of cluster c (1..n_clu) lie in cluster d (0: in no cluster).  knn: k x n int64
+ * 1-based spike ids (miKnn, the batch call's output); clu[n]: cluster per spike (<= 0: none).
  * counts (n_clu x (n_clu+1), uint64) is overwritten.  n_clu <= 4096.  opt->mem: host or device. */
-irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int32_t* knn, const int32_t* clu,
+irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int64_t* knn, const int32_t* clu,
                                 int32_t n_clu, uint64_t* counts, const irc_options* opt);
 
 #ifdef __cplusplus

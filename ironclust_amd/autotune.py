@@ -1,11 +1,12 @@
-"""Autotuner for irc_rho_delta_batch: picks the screen's over-fetch depth (k_over), its
-persistent grid (waves_per_cu), the threshold above which a block keeps 64 list slots
-(large_ncand), the histogram pre-pass and the smallest query group screened per wavefront
-(tile: the queue granularity) by timing the real call on the caller's data, and caches the
-choice in a JSON file keyed by the shape of the workload.
+"""Autotuner front end: irc_autotune (the library's tuner, include/irc_hip.h) on the caller's
+device-resident batch, with its choice cached in a JSON file keyed by the workload shape so a later
+process starts from it.
 
-Every candidate configuration must reproduce the default configuration's outputs bit for bit
-(they only change how the exact result is reached); a configuration that does not is dropped.
+The library times irc_rho_delta_batch with each candidate setting of the screen's over-fetch depth
+(k_over), persistent grid (waves_per_cu), smallest screened query group (tile: the queue
+granularity) and deep-list threshold (large_ncand), and keeps the fastest whose outputs equal the
+base setting's bit for bit (the settings only change how the exact result is reached).  Inside one
+process, IRC_FLAG_AUTOTUNE applies the context's own choice without this module.
 """
 import json
 import os
@@ -13,19 +14,6 @@ import os
 from . import libirc
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "irc_hip_autotune.json")
-
-GRID = {
-    libirc.PREC_BF16_SCREEN: [
-        dict(k_over=48), dict(k_over=40), dict(k_over=64),
-        dict(k_over=48, waves_per_cu=16), dict(k_over=48, flags=2),
-        dict(k_over=48, large_ncand=4096), dict(k_over=48, tile=8), dict(k_over=48, tile=16),
-        dict(k_over=48, tile=24), dict(k_over=40, tile=16),
-    ],
-    libirc.PREC_FP32: [
-        dict(k_over=40), dict(k_over=32), dict(k_over=48),
-        dict(k_over=40, waves_per_cu=16), dict(k_over=40, tile=8), dict(k_over=40, tile=16),
-    ],
-}
 
 
 def signature(dev, precision):
@@ -55,39 +43,18 @@ def _save(path, data):
     os.replace(tmp, path)
 
 
-def tune(ctx, dev, out, precision=libirc.PREC_BF16_SCREEN, reps=2, cache=DEFAULT_CACHE, grid=None, force=False):
+def tune(ctx, dev, out, precision=libirc.PREC_BF16_SCREEN, cache=DEFAULT_CACHE, force=False):
     """Return (options dict, report).  dev/out: device tensors as for Context.rho_delta_batch_device.
-    Timings are the library's own device time (HIP events around the call's kernels)."""
-    import torch
+    The library's trial times are its own device time (HIP events around each call's kernels)."""
     key = signature(dev, precision)
     db = _load(cache) if cache else {}
     if not force and key in db:
         return dict(db[key]["options"]), {"cached": True, "key": key}
-    cands = grid or GRID.get(precision, [dict()])
-    ref = None
-    report = []
-    best = None
-    for c in cands:
-        opts = libirc.make_options(precision=precision, **c)
-        ctx.rho_delta_batch_device(dev, out, opts)          # warm-up (buffers, code objects)
-        torch.cuda.synchronize()
-        if ref is None:
-            ref = {kk: v.clone() for kk, v in out.items()}
-            same = True
-        else:
-            same = all(torch.equal(out[kk], ref[kk]) for kk in ref)
-        ms = []
-        for _ in range(reps):
-            ctx.rho_delta_batch_device(dev, out, opts)
-            ms.append(ctx.stats()["ms_total"])
-        t = min(ms)
-        report.append({"options": c, "ms": t, "identical": same})
-        if same and (best is None or t < best[1]):
-            best = (c, t)
+    choice, rep = ctx.autotune_device(dev, out, libirc.make_options(precision=precision))
     if cache:
-        db[key] = {"options": best[0], "ms": best[1], "trials": report}
+        db[key] = {"options": choice, "ms": rep["ms_best"], "trials": rep["trials"]}
         try:
             _save(cache, db)
         except OSError:
             pass                      # a read-only home: the choice still applies to this run
-    return dict(best[0]), {"cached": False, "key": key, "trials": report}
+    return choice, dict(rep, cached=False, key=key)

@@ -51,12 +51,19 @@ struct irc_ctx {
     DevBuf b_plan;                                         // device-side work planning
     DevBuf b_drift;                                        // drift linkage scratch
     DevBuf b_assign;                                       // chain labelling scratch
+    DevBuf b_siteseg;     // per-site segment tables (IRC_FLAG_SITE_LAUNCHES)
+    DevBuf b_tune;        // the autotuner's copy of the base setting's outputs
     DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
-                          // re-rank, [2] pairs, [3] queries whose denser neighbour was outside the kNN list
+                          // re-rank, [2] pairs, [3] queries the re-rank's second pass redid, [4] queries
+                          // whose denser neighbour was outside the kNN list
     bool dstat_screen = false;  // [0], [1] hold this call's screen counters
     bool capturing = false;     // the stream is being captured into a graph: no events, no host copies
     hipGraphExec_t graph_exec = nullptr;  // the batch call's device phase, for replays
-    uint64_t graph_key = 0;
+    uint64_t graph_key = 0;     // descriptor hash of the captured call
+    uint64_t buf_gen = 0;       // bumped whenever a device buffer is (re)allocated
+    uint64_t graph_gen = 0;     // buf_gen when the graph was captured: a replay needs it unchanged
+    uint64_t desc_key = 0;      // descriptor whose runs/queries/links the planning buffers hold (0: none)
+    std::vector<std::pair<uint64_t, irc_options>> tuned;   // autotuned options per workload shape
 };
 
 namespace {
@@ -102,6 +109,8 @@ irc_status ensure(irc_ctx* ctx, DevBuf& b, size_t bytes) {
         return fail(ctx, IRC_E_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
     }
     b.cap = want;
+    // kernels captured in a graph hold the old address: a replay must not run on it
+    ++ctx->buf_gen;
     return IRC_OK;
 }
 
@@ -155,6 +164,7 @@ struct HostPlan {
 // Upload a host plan into the context buffers and return the device view.
 irc_status upload_plan(irc_ctx* ctx, const HostPlan& h, DevPlan& p) {
     const int64_t n = h.n_cols;
+    ctx->desc_key = 0;   // the batch call's planning buffers are overwritten below
     CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n * h.DP));
     CHK(ensure(ctx, ctx->b_key, sizeof(int) * (size_t)n));
     CHK(ensure(ctx, ctx->b_ret, sizeof(int) * (size_t)n));
@@ -258,35 +268,39 @@ irc_status plan_site(irc_ctx* ctx, const irc_site_problem* pr, HostPlan& h) {
     return IRC_OK;
 }
 
-// Plan of an A/B problem: chunk 0 = B (candidates), chunk 1 = A (queries), link 0 -> 1.
-void plan_ab(int D, int64_t nB, const float* B, int64_t nA, const float* A, int k, HostPlan& h) {
+// Plan of an A/B problem: chunk 0 = B (candidates), chunk 1 = A (queries), link 0 -> 1.  With IB,
+// chunk 0 holds the listed columns IB[j] - 1 of B in list order (key and reported index: the column).
+void plan_ab(int D, int64_t nB, const float* B, int64_t nA, const float* A, int k, HostPlan& h,
+             const int32_t* IB = nullptr, int64_t nIB = 0) {
+    const int64_t n0 = IB ? nIB : nB;
     h.D = D;
     h.DP = pad_dim(D);
     h.C = 2;
     h.k = k;
-    h.n_cols = nB + nA;
+    h.n_cols = n0 + nA;
     h.feat.assign((size_t)h.n_cols * h.DP, 0.0f);
     h.key.resize(h.n_cols);
     h.ret.resize(h.n_cols);
-    for (int64_t j = 0; j < nB; ++j) {
-        memcpy(&h.feat[(size_t)j * h.DP], B + (size_t)j * D, sizeof(float) * D);
-        h.key[j] = (int)j;
-        h.ret[j] = (int)(j + 1);
+    for (int64_t j = 0; j < n0; ++j) {
+        const int64_t col = IB ? (int64_t)IB[j] - 1 : j;
+        memcpy(&h.feat[(size_t)j * h.DP], B + (size_t)col * D, sizeof(float) * D);
+        h.key[j] = (int)col;
+        h.ret[j] = (int)(col + 1);
     }
     for (int64_t i = 0; i < nA; ++i) {
-        memcpy(&h.feat[(size_t)(nB + i) * h.DP], A + (size_t)i * D, sizeof(float) * D);
-        h.key[nB + i] = (int)(nB + i);
-        h.ret[nB + i] = 0;
+        memcpy(&h.feat[(size_t)(n0 + i) * h.DP], A + (size_t)i * D, sizeof(float) * D);
+        h.key[n0 + i] = (int)(n0 + i);
+        h.ret[n0 + i] = 0;
     }
-    h.runb = {0, (int)nB, (int)nB, (int)(nB + nA)};
-    h.runl = {(int)nB, 0, (int)nA, 0};
+    h.runb = {0, (int)n0, (int)n0, (int)(n0 + nA)};
+    h.runl = {(int)n0, 0, (int)nA, 0};
     h.lptr = {0, 0, 1};
     h.lidx = {0};
     h.qcol.resize(nA);
     h.qblk.assign(nA, 1);
     h.qout.resize(nA);
     for (int64_t i = 0; i < nA; ++i) {
-        h.qcol[i] = (int)(nB + i);
+        h.qcol[i] = (int)(n0 + i);
         h.qout[i] = (int)i;
     }
 }
@@ -475,7 +489,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.n_items = n_items_cap;
     a.kprime = kprime;
     a.large_ncand = large_ncand;
-    a.hist = (ctx->flags & 2) ? 1 : 0;
+    a.hist = 0;
     a.eta = irc::screen_eta(sp, p.D, KP);
     a.n_q = nq;
     a.cand_n = ptr<int>(ctx->b_cand_n);
@@ -485,9 +499,37 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.fallback = fb;
     a.n_fallback = nfb;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
-    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
-    HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
-    if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
+    if (dw && (ctx->flags & IRC_FLAG_SITE_LAUNCHES)) {
+        // one launch per site over that site's items (blocks are site-major, so a site's items are
+        // contiguous): the item offsets come back to the host once
+        const int S = dw->S, C = dw->C;
+        const int64_t nb = (int64_t)S * C;
+        std::vector<int> ioff((size_t)nb);
+        int64_t tot = 0;
+        HIPCHK(hipMemcpyAsync(ioff.data(), pb.item_off, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&tot, pb.totals, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<int> segs((size_t)S * 9);
+        for (int s = 0; s < S; ++s) {
+            const int lo = ioff[(size_t)s * C], hi = s + 1 < S ? ioff[(size_t)(s + 1) * C] : (int)tot;
+            for (int j = 0; j <= 8; ++j) segs[(size_t)s * 9 + j] = lo + (int)((int64_t)(hi - lo) * j / 8);
+        }
+        CHK(ensure(ctx, ctx->b_siteseg, sizeof(int) * segs.size()));
+        HIPCHK(hipMemcpyAsync(ctx->b_siteseg.p, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipEventRecord(ctx->ev[4], st));
+        for (int s = 0; s < S; ++s) {
+            const int n = segs[(size_t)s * 9 + 8] - segs[(size_t)s * 9];
+            if (n <= 0) continue;
+            const int g = std::min(n, persistent_grid(ctx, ctx->waves_per_cu));
+            HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, g, seg_ctr, ptr<int>(ctx->b_siteseg) + (size_t)s * 9));
+        }
+        HIPCHK(hipEventRecord(ctx->ev[5], st));
+    } else {
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
+        HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
+        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
+    }
     int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
     HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, seg_ctr + 20));
@@ -504,6 +546,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
 irc_status collect_stats(irc_ctx* ctx, bool events) {
     int64_t d[8] = {0};
     if (ctx->b_dstat.p) HIPCHK(hipMemcpy(d, ctx->b_dstat.p, sizeof(d), hipMemcpyDeviceToHost));
+    ctx->stats.n_delta_full = d[4];   // batch calls only (the kNN pass zeroes the counters)
     if (ctx->dstat_screen) {
         ctx->stats.n_screen_fallback = d[1];
         ctx->stats.n_rerank_overflow = d[1] - d[0];
@@ -517,6 +560,15 @@ irc_status collect_stats(irc_ctx* ctx, bool events) {
         ctx->stats.ms_screen = event_ms(ctx->ev[4], ctx->ev[5]);
     }
     return IRC_OK;
+}
+
+// Workload shape the autotuner's choice is kept under: precision, D, k, sites, chunks, N / 65536.
+uint64_t tune_signature(const irc_batch* b, int prec) {
+    const int64_t v[] = {prec, b->D, b->k, b->n_sites, b->n_drift, b->n_spikes >> 16};
+    uint64_t h = 1469598103934665603ull;
+    const unsigned char* c = reinterpret_cast<const unsigned char*>(v);
+    for (size_t i = 0; i < sizeof(v); ++i) h = (h ^ c[i]) * 1099511628211ull;
+    return h;
 }
 
 // MATLAB round(): half away from zero
@@ -619,7 +671,7 @@ void irc_ctx_destroy(irc_ctx* ctx) {
                      &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
                      &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
                      &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan, &ctx->b_dstat,
-                     &ctx->b_drift, &ctx->b_assign};
+                     &ctx->b_drift, &ctx->b_assign, &ctx->b_siteseg, &ctx->b_tune};
     for (DevBuf* b : all)
         if (b->p) (void)hipFree(b->p);
     for (auto& b : ctx->b_scr)
@@ -730,6 +782,25 @@ irc_status irc_delta_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc
 // ---------------------------------------------------------------------------
 irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_options* opt, irc_outputs* out) {
     if (!ctx) return IRC_E_ARG;
+    if (opt && (opt->flags & IRC_FLAG_AUTOTUNE) && !opt->force_fail && b) {
+        // the context's tuned setting for this workload shape (tuned now when there is none)
+        irc_options o = *opt;
+        o.flags &= ~IRC_FLAG_AUTOTUNE;
+        const uint64_t sig = tune_signature(b, resolve_precision(&o));
+        const irc_options* hit = nullptr;
+        for (const auto& e : ctx->tuned)
+            if (e.first == sig) hit = &e.second;
+        irc_options t;
+        if (!hit) {
+            CHK(irc_autotune(ctx, b, &o, out, &t, nullptr));
+            hit = &t;
+        }
+        o.k_over = hit->k_over;
+        o.waves_per_cu = hit->waves_per_cu;
+        o.tile = hit->tile;
+        o.large_ncand = hit->large_ncand;
+        return irc_rho_delta_batch(ctx, b, &o, out);
+    }
     CHK(check_opt(ctx, opt));
     if (!b || !out || !b->F || !b->col_spike || !b->site_ptr || !b->site_n_primary || !b->lim_drift || !b->link)
         return fail(ctx, IRC_E_ARG, "null pointer in irc_batch");
@@ -758,57 +829,50 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     for (int c = 0; c < C; ++c)
         if (b->lim_drift[c + 1] < b->lim_drift[c]) return fail(ctx, IRC_E_ARG, "lim_drift not monotone");
     if (!links_have_diagonal(C, b->link)) return fail(ctx, IRC_E_ARG, "link(c,c) must be set for every chunk");
-    std::vector<int> lptr, lidx;
-    build_links(C, b->link, lptr, lidx);
 
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int DP = pad_dim(D);
-    // small descriptors in one buffer: site_ptr[S+1], site_np[S], prim[S+1], lim[C+1]
-    const size_t n_small = (size_t)(S + 1) + S + (S + 1) + (C + 1);
-    CHK(ensure(ctx, ctx->b_small, sizeof(int64_t) * n_small));
-    int64_t* d_site_ptr = ptr<int64_t>(ctx->b_small);
-    int64_t* d_site_np = d_site_ptr + (S + 1);
-    int64_t* d_prim = d_site_np + S;
-    int64_t* d_lim = d_prim + (S + 1);
-    {
-        std::vector<int64_t> smallv(n_small);
-        memcpy(&smallv[0], b->site_ptr, sizeof(int64_t) * (S + 1));
-        memcpy(&smallv[S + 1], b->site_n_primary, sizeof(int64_t) * S);
-        memcpy(&smallv[2 * S + 1], prim.data(), sizeof(int64_t) * (S + 1));
-        memcpy(&smallv[3 * S + 2], b->lim_drift, sizeof(int64_t) * (C + 1));
-        HIPCHK(hipMemcpyAsync(d_site_ptr, smallv.data(), sizeof(int64_t) * n_small, hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
+    const bool graph = opt && (opt->flags & IRC_FLAG_GRAPH);
+    if (graph && (opt->flags & IRC_FLAG_SITE_LAUNCHES))
+        return fail(ctx, IRC_E_ARG, "IRC_FLAG_SITE_LAUNCHES cannot be captured in a graph (IRC_FLAG_GRAPH)");
+    // descriptor of the call: device pointers, sizes, options and the host descriptors' contents
+    uint64_t h = 1469598103934665603ull;
+    if (graph) {
+        auto mix = [&](const void* v, size_t n) {
+            const unsigned char* c = reinterpret_cast<const unsigned char*>(v);
+            for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+        };
+        const void* ptrs[] = {b->F, b->col_spike, out->rho, out->delta, out->nneigh, out->knn, out->dk};
+        const int64_t sizes[] = {N, S, C, D, k, n_cols, dev ? 1 : 0};
+        mix(dev ? ptrs : ptrs + 7, dev ? sizeof(ptrs) : 0);
+        const int64_t want[] = {out->knn != nullptr, out->dk != nullptr};
+        mix(want, sizeof(want));
+        mix(sizes, sizeof(sizes));
+        mix(opt, sizeof(*opt));
+        mix(b->site_ptr, sizeof(int64_t) * (S + 1));
+        mix(b->site_n_primary, sizeof(int64_t) * S);
+        mix(b->lim_drift, sizeof(int64_t) * (C + 1));
+        mix(b->link, (size_t)C * C);
+        if (h == 0) h = 1;
     }
-    CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * lptr.size()));
-    CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (lidx.size() + 1)));
-    HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, st));
-    if (!lidx.empty()) HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, st));
+    // The replay needs no set-up when the captured graph still matches: same descriptor, no context
+    // buffer reallocated since the capture (buf_gen), and the shared planning buffers still hold this
+    // descriptor's runs and queries (no other call has written them since, desc_key).
+    const bool fast = graph && ctx->graph_exec && ctx->graph_key == h && ctx->graph_gen == ctx->buf_gen &&
+                      ctx->desc_key == h;
+    ctx->desc_key = 0;
 
     const float* dF = b->F;
     const int* dspk = b->col_spike;
     if (!dev) {
         CHK(ensure(ctx, ctx->b_F, sizeof(float) * (size_t)n_cols * D));
-        CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
-        HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(ctx->b_colspk.p, b->col_spike, sizeof(int) * (size_t)n_cols, hipMemcpyHostToDevice, st));
         dF = ptr<float>(ctx->b_F);
         dspk = ptr<int>(ctx->b_colspk);
     }
-    CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n_cols * DP));
-    CHK(ensure(ctx, ctx->b_runb, sizeof(int) * (size_t)S * C * 2));
-    CHK(ensure(ctx, ctx->b_runl, sizeof(int) * (size_t)S * C * 2));
-    CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * (size_t)N));
-    CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * (size_t)N));
-    CHK(ensure(ctx, ctx->b_qout, sizeof(int) * (size_t)N));
-    CHK(ensure(ctx, ctx->b_flags, sizeof(int) * 4));
-    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)N * k));
-    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
-    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
-    CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
     float *o_rho, *o_delta, *o_dk = nullptr;
     int64_t* o_nn;
-    int32_t* o_knn = nullptr;
+    int64_t* o_knn = nullptr;
     if (dev) {
         o_rho = out->rho;
         o_delta = out->delta;
@@ -823,105 +887,140 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         o_delta = ptr<float>(ctx->b_out_delta);
         o_nn = ptr<int64_t>(ctx->b_out_nn);
         if (out->knn) {
-            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int32_t) * (size_t)N * k));
-            o_knn = ptr<int32_t>(ctx->b_out_knn);
+            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int64_t) * (size_t)N * k));
+            o_knn = ptr<int64_t>(ctx->b_out_knn);
         }
         if (out->dk) {
             CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)N));
             o_dk = ptr<float>(ctx->b_out_dk);
         }
     }
-    DevPlan p;
-    p.feat = ptr<float>(ctx->b_feat);
-    p.DP = DP;
-    p.D = D;
-    p.C = C;
-    p.k = k;
-    p.col_key = dspk;
-    p.col_ret = dspk;
-    p.run_beg = ptr<int>(ctx->b_runb);
-    p.run_len = ptr<int>(ctx->b_runl);
-    p.link_ptr = ptr<int>(ctx->b_lptr);
-    p.link_idx = ptr<int>(ctx->b_lidx);
-    p.q_col = ptr<int>(ctx->b_qcol);
-    p.q_blk = ptr<int>(ctx->b_qblk);
-    p.q_out = ptr<int>(ctx->b_qout);
-    p.nq = (int)N;
-    p.n_blocks = S * C;
-    int* d_flags = ptr<int>(ctx->b_flags);
-    int* d_nunres = ptr<int>(ctx->b_unres) + N;
-    const int nq = (int)N;
-
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = N;
-    HIPCHK(irc::launch_fill_int(st, d_flags, 3, 0));
-    HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
-                                  ptr<int>(ctx->b_runl)));
-    HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
-                                     ptr<int>(ctx->b_qcol), ptr<int>(ctx->b_qblk), ptr<int>(ctx->b_qout), d_flags));
-    HIPCHK(irc::launch_max_ncand(st, S, C, p.link_ptr, p.link_idx, p.run_len, d_flags + 2));
-    int hflags[3] = {0, 0, 0};
-    HIPCHK(hipMemcpyAsync(hflags, d_flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const int flags = hflags[0];
-    if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
-    if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
-    if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
-    // the work list is built on the device from the block runs (no host round trip)
-    DevWork dw;
-    dw.S = S;
-    dw.C = C;
-    dw.site_ptr = d_site_ptr;
-    dw.prim = d_prim;
-    dw.max_ncand = hflags[2];
-    // the device phase: kNN (plan, pack, screen, re-rank, exact), rho, delta.  With
-    // IRC_FLAG_GRAPH it is captured once into a HIP graph and replayed while the call's
-    // descriptor (pointers, sizes, options, host descriptor contents) is unchanged.
-    auto device_phase = [&]() -> irc_status {
-        HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
-        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
-        CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols, nullptr,
-                    &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), ptr<float>(ctx->b_feat)));
-        HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho, o_dk,
-                                      o_knn));
-        if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
-        HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
-        HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol), o_delta,
-                                          o_nn, ptr<int>(ctx->b_unres), d_nunres));
-        HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
-                                       ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
-        HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->b_dstat) + 3));
-        return IRC_OK;
-    };
-    const bool graph = opt && (opt->flags & IRC_FLAG_GRAPH);
     bool replayed = false;
-    if (graph) {
-        uint64_t h = 1469598103934665603ull;
-        auto mix = [&](const void* v, size_t n) {
-            const unsigned char* c = reinterpret_cast<const unsigned char*>(v);
-            for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    if (fast) {
+        // features in, then the captured device phase; no host round trip before the final one
+        HIPCHK(hipEventRecord(ctx->ev[0], st));
+        if (!dev)
+            HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(ctx->ev[1], st));
+        HIPCHK(hipGraphLaunch(ctx->graph_exec, st));
+        replayed = true;
+    } else {
+        std::vector<int> lptr, lidx;
+        build_links(C, b->link, lptr, lidx);
+        // small descriptors in one buffer: site_ptr[S+1], site_np[S], prim[S+1], lim[C+1]
+        const size_t n_small = (size_t)(S + 1) + S + (S + 1) + (C + 1);
+        CHK(ensure(ctx, ctx->b_small, sizeof(int64_t) * n_small));
+        CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * lptr.size()));
+        CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (lidx.size() + 1)));
+        CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n_cols * DP));
+        CHK(ensure(ctx, ctx->b_runb, sizeof(int) * (size_t)S * C * 2));
+        CHK(ensure(ctx, ctx->b_runl, sizeof(int) * (size_t)S * C * 2));
+        CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_qout, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_flags, sizeof(int) * 4));
+        CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)N * k));
+        CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
+        CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
+        CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
+        if (!dev) CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
+        int64_t* d_site_ptr = ptr<int64_t>(ctx->b_small);
+        int64_t* d_site_np = d_site_ptr + (S + 1);
+        int64_t* d_prim = d_site_np + S;
+        int64_t* d_lim = d_prim + (S + 1);
+        {
+            std::vector<int64_t> smallv(n_small);
+            memcpy(&smallv[0], b->site_ptr, sizeof(int64_t) * (S + 1));
+            memcpy(&smallv[S + 1], b->site_n_primary, sizeof(int64_t) * S);
+            memcpy(&smallv[2 * S + 1], prim.data(), sizeof(int64_t) * (S + 1));
+            memcpy(&smallv[3 * S + 2], b->lim_drift, sizeof(int64_t) * (C + 1));
+            HIPCHK(hipMemcpyAsync(d_site_ptr, smallv.data(), sizeof(int64_t) * n_small, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, st));
+            if (!lidx.empty())
+                HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, st));
+            // the host vectors are released at the end of this scope
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        if (!dev) {
+            HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->b_colspk.p, b->col_spike, sizeof(int) * (size_t)n_cols, hipMemcpyHostToDevice, st));
+        }
+        DevPlan p;
+        p.feat = ptr<float>(ctx->b_feat);
+        p.DP = DP;
+        p.D = D;
+        p.C = C;
+        p.k = k;
+        p.col_key = dspk;
+        p.col_ret = dspk;
+        p.run_beg = ptr<int>(ctx->b_runb);
+        p.run_len = ptr<int>(ctx->b_runl);
+        p.link_ptr = ptr<int>(ctx->b_lptr);
+        p.link_idx = ptr<int>(ctx->b_lidx);
+        p.q_col = ptr<int>(ctx->b_qcol);
+        p.q_blk = ptr<int>(ctx->b_qblk);
+        p.q_out = ptr<int>(ctx->b_qout);
+        p.nq = (int)N;
+        p.n_blocks = S * C;
+        int* d_flags = ptr<int>(ctx->b_flags);
+        int* d_nunres = ptr<int>(ctx->b_unres) + N;
+        const int nq = (int)N;
+
+        HIPCHK(irc::launch_fill_int(st, d_flags, 3, 0));
+        HIPCHK(hipEventRecord(ctx->ev[0], st));
+        HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
+                                      ptr<int>(ctx->b_runl)));
+        HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
+                                         ptr<int>(ctx->b_qcol), ptr<int>(ctx->b_qblk), ptr<int>(ctx->b_qout), d_flags));
+        HIPCHK(irc::launch_max_ncand(st, S, C, p.link_ptr, p.link_idx, p.run_len, d_flags + 2));
+        int hflags[3] = {0, 0, 0};
+        HIPCHK(hipMemcpyAsync(hflags, d_flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        const int flags = hflags[0];
+        if (flags & 1) return fail(ctx, IRC_E_ARG, "col_spike holds ids outside 1..n_spikes");
+        if (flags & 2) return fail(ctx, IRC_E_ARG, "spike ids outside lim_drift");
+        if (flags & 4) return fail(ctx, IRC_E_ARG, "spike ids not ascending within a site part");
+        // the work list is built on the device from the block runs (no host round trip)
+        DevWork dw;
+        dw.S = S;
+        dw.C = C;
+        dw.site_ptr = d_site_ptr;
+        dw.prim = d_prim;
+        dw.max_ncand = hflags[2];
+        // the device phase: kNN (plan, pack, screen, re-rank, exact), rho, delta.  With
+        // IRC_FLAG_GRAPH it is captured once into a HIP graph and replayed while the call's
+        // descriptor is unchanged and no context buffer has been reallocated since the capture.
+        auto device_phase = [&]() -> irc_status {
+            HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
+            if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
+            CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols,
+                        nullptr, &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), ptr<float>(ctx->b_feat)));
+            HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho,
+                                          o_dk, nullptr, o_knn));
+            if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
+            HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
+            HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol),
+                                              o_delta, o_nn, ptr<int>(ctx->b_unres), d_nunres));
+            HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
+                                           ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
+            HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->b_dstat) + 4));
+            return IRC_OK;
         };
-        const void* ptrs[] = {dF, dspk, o_rho, o_delta, o_nn, o_knn, o_dk};
-        const int64_t sizes[] = {N, S, C, D, k, n_cols, dev ? 1 : 0};
-        mix(ptrs, sizeof(ptrs));
-        mix(sizes, sizeof(sizes));
-        mix(opt, sizeof(*opt));
-        mix(b->site_ptr, sizeof(int64_t) * (S + 1));
-        mix(b->site_n_primary, sizeof(int64_t) * S);
-        mix(b->lim_drift, sizeof(int64_t) * (C + 1));
-        mix(b->link, (size_t)C * C);
-        if (ctx->graph_exec && ctx->graph_key == h) {
+        if (graph && ctx->graph_exec && ctx->graph_key == h && ctx->graph_gen == ctx->buf_gen) {
+            // the set-up above rewrote the shared planning buffers; the graph itself still matches
             HIPCHK(hipEventRecord(ctx->ev[1], st));
             HIPCHK(hipGraphLaunch(ctx->graph_exec, st));
             replayed = true;
-        } else {
+        } else if (graph) {
             CHK(device_phase());
             // record the same work (all buffers now exist) for the next call with this descriptor
             if (ctx->graph_exec) {
                 (void)hipGraphExecDestroy(ctx->graph_exec);
                 ctx->graph_exec = nullptr;
             }
+            const uint64_t gen0 = ctx->buf_gen;
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             ctx->capturing = true;
@@ -933,6 +1032,11 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
                 return cs;
             }
             if (ce != hipSuccess) return fail(ctx, IRC_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            if (ctx->buf_gen != gen0) {
+                // a buffer grew while capturing (it cannot: the eager run sized them all): drop the graph
+                (void)hipGraphDestroy(g);
+                return fail(ctx, IRC_E_HIP, "a context buffer was reallocated during graph capture");
+            }
             const hipError_t ie = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (ie != hipSuccess) {
@@ -940,9 +1044,10 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
                 return fail(ctx, IRC_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
             }
             ctx->graph_key = h;
+            ctx->graph_gen = ctx->buf_gen;
+        } else {
+            CHK(device_phase());
         }
-    } else {
-        CHK(device_phase());
     }
     HIPCHK(hipEventRecord(ctx->ev[3], st));
 
@@ -950,43 +1055,196 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         HIPCHK(hipMemcpyAsync(out->rho, o_rho, sizeof(float) * N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(out->delta, o_delta, sizeof(float) * N, hipMemcpyDeviceToHost, st));
         HIPCHK(hipMemcpyAsync(out->nneigh, o_nn, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st));
-        if (out->knn) HIPCHK(hipMemcpyAsync(out->knn, o_knn, sizeof(int32_t) * (size_t)N * k, hipMemcpyDeviceToHost, st));
+        if (out->knn) HIPCHK(hipMemcpyAsync(out->knn, o_knn, sizeof(int64_t) * (size_t)N * k, hipMemcpyDeviceToHost, st));
         if (out->dk) HIPCHK(hipMemcpyAsync(out->dk, o_dk, sizeof(float) * N, hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
     CHK(collect_stats(ctx, !replayed));
-    {
-        int64_t d3 = 0;
-        HIPCHK(hipMemcpy(&d3, ptr<int64_t>(ctx->b_dstat) + 3, sizeof(d3), hipMemcpyDeviceToHost));
-        ctx->stats.n_delta_full = d3;
-    }
     if (!replayed) {
         ctx->stats.ms_knn = event_ms(ctx->ev[1], ctx->ev[2]);
         ctx->stats.ms_delta = event_ms(ctx->ev[2], ctx->ev[3]);
     }
     ctx->stats.ms_graph = replayed ? event_ms(ctx->ev[1], ctx->ev[3]) : 0.0;
     ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[3]);
+    // the shared planning buffers (runs, queries, links, small descriptors) now hold this descriptor
+    ctx->desc_key = graph ? h : 0;
     return IRC_OK;
 }
 
 // ---------------------------------------------------------------------------
-irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A, int32_t k,
-                      float* K_out, int32_t* I_out) {
+irc_status irc_autotune(irc_ctx* ctx, const irc_batch* b, const irc_options* base, irc_outputs* out,
+                        irc_options* best, irc_tune_report* rep) {
     if (!ctx) return IRC_E_ARG;
-    if (D <= 0 || k <= 0 || nB <= 0 || nA < 0 || !B || (nA > 0 && (!A || !K_out)))
+    if (!b || !base || !out || !best) return fail(ctx, IRC_E_ARG, "null argument in irc_autotune");
+    if (base->force_fail) return fail(ctx, IRC_E_FORCED, "failure forced by irc_options.force_fail");
+    irc_options o0 = *base;
+    o0.flags &= ~(IRC_FLAG_AUTOTUNE | IRC_FLAG_GRAPH | IRC_FLAG_SITE_LAUNCHES);
+    const int prec = resolve_precision(&o0);
+    const bool dev = o0.mem == IRC_MEM_DEVICE;
+    // candidates: the base setting, then one parameter changed at a time
+    std::vector<irc_options> cand{o0};
+    if (prec == IRC_PREC_BF16_SCREEN || prec == IRC_PREC_FP32) {
+        const int kd = o0.k_over > 0 ? o0.k_over : (prec == IRC_PREC_FP32 ? 40 : 48);
+        for (int kv : {32, 40, 48, 64})
+            if (kv != kd && kv >= b->k) { irc_options o = o0; o.k_over = kv; cand.push_back(o); }
+        const int wd = o0.waves_per_cu > 0 ? o0.waves_per_cu : 8;
+        for (int wv : {4, 16})
+            if (wv != wd) { irc_options o = o0; o.waves_per_cu = wv; cand.push_back(o); }
+        for (int tv : {8, 16, 24})
+            if (tv != o0.tile) { irc_options o = o0; o.tile = tv; cand.push_back(o); }
+        const int ld = o0.large_ncand > 0 ? o0.large_ncand : 8192;
+        for (int lv : {4096, 16384})
+            if (lv != ld) { irc_options o = o0; o.large_ncand = lv; cand.push_back(o); }
+    }
+    if (cand.size() > 15) cand.resize(15);   // one slot of the report is kept for the combined setting
+    const int64_t N = b->n_spikes;
+    const int k = b->k;
+    // the outputs the trials are compared against: base's, kept in the context (device) or on the host
+    const size_t sz[5] = {sizeof(float) * (size_t)N, sizeof(float) * (size_t)N, sizeof(int64_t) * (size_t)N,
+                          out->knn ? sizeof(int64_t) * (size_t)N * k : 0, out->dk ? sizeof(float) * (size_t)N : 0};
+    void* optr[5] = {out->rho, out->delta, out->nneigh, out->knn, out->dk};
+    size_t off[6] = {0};
+    for (int i = 0; i < 5; ++i) off[i + 1] = off[i] + ((sz[i] + 255) & ~(size_t)255);
+    std::vector<char> host_ref;
+    auto run = [&](const irc_options& o, double& ms) -> irc_status {
+        ms = 1e30;
+        for (int r = 0; r < 3; ++r) {   // the first run also warms the buffers and code objects up
+            const irc_status st = irc_rho_delta_batch(ctx, b, &o, out);
+            if (st != IRC_OK) return st;
+            if (r > 0) ms = std::min(ms, ctx->stats.ms_total);
+        }
+        return IRC_OK;
+    };
+    double ms0 = 0.0;
+    CHK(run(o0, ms0));
+    hipStream_t st = ctx->stream;
+    if (dev) {
+        CHK(ensure(ctx, ctx->b_tune, off[5] + 256));
+        for (int i = 0; i < 5; ++i)
+            if (sz[i]) HIPCHK(hipMemcpyAsync((char*)ctx->b_tune.p + off[i], optr[i], sz[i], hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } else {
+        host_ref.resize(off[5]);
+        for (int i = 0; i < 5; ++i)
+            if (sz[i]) memcpy(host_ref.data() + off[i], optr[i], sz[i]);
+    }
+    auto same = [&](bool& eq) -> irc_status {
+        eq = true;
+        if (dev) {
+            unsigned long long* cnt = reinterpret_cast<unsigned long long*>((char*)ctx->b_tune.p + off[5]);
+            HIPCHK(irc::launch_zero_i64(st, reinterpret_cast<int64_t*>(cnt), 1));
+            for (int i = 0; i < 5; ++i)
+                if (sz[i])
+                    HIPCHK(irc::launch_count_diff(st, reinterpret_cast<const uint32_t*>((char*)ctx->b_tune.p + off[i]),
+                                                  reinterpret_cast<const uint32_t*>(optr[i]), (int64_t)(sz[i] / 4), cnt));
+            unsigned long long nd = 0;
+            HIPCHK(hipMemcpyAsync(&nd, cnt, sizeof(nd), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            eq = nd == 0;
+        } else {
+            for (int i = 0; i < 5 && eq; ++i)
+                if (sz[i]) eq = memcmp(host_ref.data() + off[i], optr[i], sz[i]) == 0;
+        }
+        return IRC_OK;
+    };
+    std::vector<double> ms(cand.size(), 1e30);
+    std::vector<int> ident(cand.size(), 0);
+    ms[0] = ms0;
+    ident[0] = 1;
+    for (size_t i = 1; i < cand.size(); ++i) {
+        if (run(cand[i], ms[i]) != IRC_OK) continue;   // an unsupported setting is not a candidate
+        bool eq = false;
+        CHK(same(eq));
+        ident[i] = eq;
+    }
+    // the combined setting: every parameter at its best single-change value
+    irc_options comb = o0;
+    auto better = [&](size_t i) { return ident[i] && ms[i] < ms0; };
+    double bk = ms0, bw = ms0, bt = ms0, bl = ms0;
+    for (size_t i = 1; i < cand.size(); ++i) {
+        if (!better(i)) continue;
+        if (cand[i].k_over != o0.k_over && ms[i] < bk) { bk = ms[i]; comb.k_over = cand[i].k_over; }
+        if (cand[i].waves_per_cu != o0.waves_per_cu && ms[i] < bw) { bw = ms[i]; comb.waves_per_cu = cand[i].waves_per_cu; }
+        if (cand[i].tile != o0.tile && ms[i] < bt) { bt = ms[i]; comb.tile = cand[i].tile; }
+        if (cand[i].large_ncand != o0.large_ncand && ms[i] < bl) { bl = ms[i]; comb.large_ncand = cand[i].large_ncand; }
+    }
+    if (memcmp(&comb, &o0, sizeof(comb)) != 0) {
+        cand.push_back(comb);
+        ms.push_back(1e30);
+        ident.push_back(0);
+        const size_t i = cand.size() - 1;
+        if (run(comb, ms[i]) == IRC_OK) {
+            bool eq = false;
+            CHK(same(eq));
+            ident[i] = eq;
+        }
+    }
+    size_t bi = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (ident[i] && ms[i] < ms[bi]) bi = i;
+    // leave base's outputs in place
+    if (dev) {
+        for (int i = 0; i < 5; ++i)
+            if (sz[i]) HIPCHK(hipMemcpyAsync(optr[i], (char*)ctx->b_tune.p + off[i], sz[i], hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    } else {
+        for (int i = 0; i < 5; ++i)
+            if (sz[i]) memcpy(optr[i], host_ref.data() + off[i], sz[i]);
+    }
+    *best = cand[bi];
+    best->flags = base->flags & ~IRC_FLAG_AUTOTUNE;
+    best->mem = base->mem;
+    const uint64_t sig = tune_signature(b, prec);
+    bool stored = false;
+    for (auto& e : ctx->tuned)
+        if (e.first == sig) { e.second = cand[bi]; stored = true; }
+    if (!stored) ctx->tuned.emplace_back(sig, cand[bi]);
+    if (rep) {
+        memset(rep, 0, sizeof(*rep));
+        rep->n_trials = (int32_t)std::min<size_t>(cand.size(), 16);
+        rep->best_trial = (int32_t)bi;
+        rep->ms_base = ms0;
+        rep->ms_best = ms[bi];
+        for (int i = 0; i < rep->n_trials; ++i) {
+            rep->k_over[i] = cand[i].k_over;
+            rep->waves_per_cu[i] = cand[i].waves_per_cu;
+            rep->tile[i] = cand[i].tile;
+            rep->large_ncand[i] = cand[i].large_ncand;
+            rep->ms[i] = ms[i];
+            rep->identical[i] = ident[i];
+        }
+    }
+    return IRC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// The A/B kNN forms: exact kNN of the columns of A among the columns of B (or the listed columns
+// IB of B).  K_all (k x nA): every listed distance; K_out (nA): the k-th; I_out (k x nA): indices.
+irc_status knn_ab_impl(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, const int32_t* IB, int64_t nIB,
+                       int64_t nA, const float* A, int32_t k, float* K_all, float* K_out, int32_t* I_out) {
+    if (!ctx) return IRC_E_ARG;
+    if (D <= 0 || k <= 0 || nB <= 0 || nA < 0 || !B || (nA > 0 && (!A || (!K_out && !K_all))))
         return fail(ctx, IRC_E_ARG, "invalid arguments");
+    if (IB && nIB <= 0) return fail(ctx, IRC_E_ARG, "empty candidate list IB");
     if (D > IRC_DMAX || k > IRC_KMAX) return fail(ctx, IRC_E_UNSUPPORTED, "D or k too large");
-    if (nA + nB >= ((int64_t)1 << 31)) return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    const int64_t n0 = IB ? nIB : nB;
+    if (nA + n0 >= ((int64_t)1 << 31) || nB >= ((int64_t)1 << 31))
+        return fail(ctx, IRC_E_UNSUPPORTED, "too many columns");
+    for (int64_t j = 0; IB && j < nIB; ++j)
+        if (IB[j] < 1 || IB[j] > nB) return fail(ctx, IRC_E_ARG, "IB[%lld]=%d outside 1..%lld", (long long)j, IB[j],
+                                                 (long long)nB);
     if (nA == 0) return IRC_OK;
     HIPCHK(hipSetDevice(ctx->device));
     HostPlan h;
-    plan_ab(D, nB, B, nA, A, k, h);
+    plan_ab(D, nB, B, nA, A, k, h, IB, nIB);
     DevPlan p;
     CHK(upload_plan(ctx, h, p));
     const int nq = (int)nA;
     CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
     CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq * (K_all ? k : 1)));
     CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
@@ -995,11 +1253,37 @@ irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64
     CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, &no_ranges, nullptr,
                 ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
     HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
-                                  ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
-    HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+                                  K_all ? nullptr : ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
+    if (K_all) {
+        HIPCHK(irc::launch_knn_dists(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_out_dk)));
+        HIPCHK(hipMemcpyAsync(K_all, ctx->b_out_dk.p, sizeof(float) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    } else {
+        HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    }
     if (I_out) HIPCHK(hipMemcpyAsync(I_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
+}
+}  // namespace
+
+extern "C" {
+
+irc_status irc_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A, int32_t k,
+                      float* K_out, int32_t* I_out) {
+    if (ctx && nA > 0 && !K_out) return fail(ctx, IRC_E_ARG, "null K_out");
+    return knn_ab_impl(ctx, D, nB, B, nullptr, 0, nA, A, k, nullptr, K_out, I_out);
+}
+
+irc_status irc_knn_ab_full(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nA, const float* A,
+                           int32_t k, float* K_out, int32_t* I_out) {
+    if (ctx && nA > 0 && (!K_out || !I_out)) return fail(ctx, IRC_E_ARG, "null output");
+    return knn_ab_impl(ctx, D, nB, B, nullptr, 0, nA, A, k, K_out, nullptr, I_out);
+}
+
+irc_status irc_knn_ab_ib(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, int64_t nIB, const int32_t* IB,
+                         int64_t nA, const float* A, int32_t k, float* K_out, int32_t* I_out) {
+    if (ctx && (!IB || (nA > 0 && !K_out))) return fail(ctx, IRC_E_ARG, "null IB or K_out");
+    return knn_ab_impl(ctx, D, nB, B, IB, nIB, nA, A, k, nullptr, K_out, I_out);
 }
 
 irc_status irc_delta_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, const float* RB, int64_t nA,
@@ -1183,7 +1467,7 @@ irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int3
     return IRC_OK;
 }
 
-irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int32_t* knn, const int32_t* clu,
+irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int64_t* knn, const int32_t* clu,
                                 int32_t n_clu, uint64_t* counts, const irc_options* opt) {
     if (!ctx) return IRC_E_ARG;
     if (n < 0 || k < 1 || n_clu < 1 || !counts || (n > 0 && (!knn || !clu)))
@@ -1196,15 +1480,15 @@ irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int32_
     const size_t nc = (size_t)n_clu * (size_t)(n_clu + 1);
     const size_t o_cnt = 0, o_clu = (sizeof(uint64_t) * nc + 255) & ~(size_t)255;
     const size_t o_knn = (o_clu + sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
-    CHK(ensure(ctx, ctx->b_assign, dev ? o_clu : o_knn + sizeof(int32_t) * (size_t)n * k));
+    CHK(ensure(ctx, ctx->b_assign, dev ? o_clu : o_knn + sizeof(int64_t) * (size_t)n * k));
     char* base = reinterpret_cast<char*>(ctx->b_assign.p);
     unsigned long long* d_cnt = dev ? reinterpret_cast<unsigned long long*>(counts)
                                     : reinterpret_cast<unsigned long long*>(base + o_cnt);
     const int32_t* d_clu = dev ? clu : reinterpret_cast<const int32_t*>(base + o_clu);
-    const int32_t* d_knn = dev ? knn : reinterpret_cast<const int32_t*>(base + o_knn);
+    const int64_t* d_knn = dev ? knn : reinterpret_cast<const int64_t*>(base + o_knn);
     if (!dev) {
         HIPCHK(hipMemcpyAsync(base + o_clu, clu, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(base + o_knn, knn, sizeof(int32_t) * (size_t)n * k, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(base + o_knn, knn, sizeof(int64_t) * (size_t)n * k, hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemsetAsync(d_cnt, 0, sizeof(uint64_t) * nc, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));

@@ -13,7 +13,7 @@ hipError_t launch_assign_chain(hipStream_t st, int64_t n, const int64_t* nneigh,
 // counts[(c - 1) * (n_clu + 1) + d] += #{(s, j): clu[s] = c, clu[knn(j, s)] = d} for clusters c >= 1,
 // with d = 0 for neighbours outside every cluster; knn is k x n column-major, 1-based.  counts must
 // be zeroed by the caller.
-hipError_t launch_isolation_counts(hipStream_t st, int64_t n, int k, const int32_t* knn, const int32_t* clu,
+hipError_t launch_isolation_counts(hipStream_t st, int64_t n, int k, const int64_t* knn, const int32_t* clu,
                                    int n_clu, unsigned long long* counts);
 
 }  // namespace irc

@@ -39,7 +39,7 @@ __global__ void k_chain_jump(int64_t n, int32_t* lab, int32_t* nxt) {
 
 // one thread per spike: its own-label neighbours are added with one atomic, the others one each
 // (well-separated clusters have few)
-__global__ void k_isolation_counts(int64_t n, int k, const int32_t* __restrict__ knn, const int32_t* __restrict__ clu,
+__global__ void k_isolation_counts(int64_t n, int k, const int64_t* __restrict__ knn, const int32_t* __restrict__ clu,
                                    int n_clu, unsigned long long* __restrict__ counts) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -48,7 +48,7 @@ __global__ void k_isolation_counts(int64_t n, int k, const int32_t* __restrict__
     unsigned long long* row = counts + (int64_t)(c - 1) * (n_clu + 1);
     unsigned long long own = 0;
     for (int j = 0; j < k; ++j) {
-        const int64_t t = (int64_t)knn[s * k + j] - 1;
+        const int64_t t = knn[s * k + j] - 1;
         int d = (t >= 0 && t < n) ? clu[t] : 0;
         if (d < 1 || d > n_clu) d = 0;
         if (d == c) ++own;
@@ -59,7 +59,7 @@ __global__ void k_isolation_counts(int64_t n, int k, const int32_t* __restrict__
 
 }  // namespace
 
-hipError_t launch_isolation_counts(hipStream_t st, int64_t n, int k, const int32_t* knn, const int32_t* clu,
+hipError_t launch_isolation_counts(hipStream_t st, int64_t n, int k, const int64_t* knn, const int32_t* clu,
                                    int n_clu, unsigned long long* counts) {
     if (n <= 0) return hipSuccess;
     k_isolation_counts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, k, knn, clu, n_clu, counts);

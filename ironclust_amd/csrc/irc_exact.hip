@@ -181,15 +181,18 @@ __global__ __launch_bounds__(64) void k_knn_exact(DevPlan p, const int* __restri
 // ---------------------------------------------------------------------------
 // Finish the kNN pass: rho (batch mode) or d_k (site mode) and the reported ids.
 //   mode 0 (batch): rho_slot[slot] = float(1/sqrt(dk2)); dk_slot (opt) = float(sqrt(dk2));
-//                   knn32[slot*k+j] = col_ret (opt, miKnn's int32)
-//   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret
+//                   knn64[slot*k+j] = col_ret (opt; int64 spike ids, miKnn after
+//                   int64(viSpk_out(miKnn_in)), matlab/irc2.m:3670)
+//   mode 1 (site) : dk_slot[slot] = float(sqrt(dk2)); knn32[slot*k+j] = col_ret (int32 column
+//                   indices, search_knn_drift_'s miKnn_in)
 // A workgroup takes 64 queries: the slot-major lists are read with coalesced loads into an LDS
 // tile, then written out query-major (k contiguous entries per reported slot).
 constexpr int FIN_Q = 64;
 
 __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode, const int* __restrict__ knn_col,
                                                     const double* __restrict__ dk2, float* __restrict__ rho_slot,
-                                                    float* __restrict__ dk_slot, int* __restrict__ knn32) {
+                                                    float* __restrict__ dk_slot, int* __restrict__ knn32,
+                                                    int64_t* __restrict__ knn64) {
     __shared__ int tile[IRC_KMAX * FIN_Q];
     const int64_t i0 = (int64_t)blockIdx.x * FIN_Q;
     const int nb = nq - i0 < FIN_Q ? (int)(nq - i0) : FIN_Q;
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode,
         if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
         if (dk_slot) dk_slot[slot] = (float)sqrt(d);
     }
-    if (!knn32) return;
+    if (!knn32 && !knn64) return;
     for (int e = t; e < k * FIN_Q; e += blockDim.x) {
         const int j = e / FIN_Q, c = e - j * FIN_Q;
         if (c < nb) tile[j * FIN_Q + c] = p.col_ret[knn_col[(int64_t)j * p.nq + i0 + c]];
@@ -211,8 +214,29 @@ __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode,
         const int c = e / k, j = e - c * k;
         const int64_t slot = p.q_out[i0 + c];
         const int v = tile[j * FIN_Q + c];
-        knn32[slot * k + j] = v;
+        if (knn64) knn64[slot * k + j] = v;
+        else knn32[slot * k + j] = v;
     }
+}
+
+// Every listed neighbour's distance (the cuda_knn_full output, cuda_knn_full.cu:97-100): one thread
+// per (query, slot), the canonical FP64 chain over the padded rows.
+__global__ void k_knn_dists(DevPlan p, int nq, const int* __restrict__ knn_col, float* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)nq * p.k) return;
+    const int q = (int)(e / p.k), j = (int)(e - (int64_t)q * p.k);
+    const float4* a = reinterpret_cast<const float4*>(p.feat + (int64_t)p.q_col[q] * p.DP);
+    const float4* b = reinterpret_cast<const float4*>(p.feat + (int64_t)knn_col[(int64_t)j * p.nq + q] * p.DP);
+    double t = 0.0;
+    for (int i = 0; i < p.DP / 4; ++i) {
+        const float4 x = a[i], y = b[i];
+        double d;
+        d = (double)x.x - (double)y.x; t = fma(d, d, t);
+        d = (double)x.y - (double)y.y; t = fma(d, d, t);
+        d = (double)x.z - (double)y.z; t = fma(d, d, t);
+        d = (double)x.w - (double)y.w; t = fma(d, d, t);
+    }
+    out[(int64_t)p.q_out[q] * p.k + j] = (float)sqrt(t);
 }
 
 // rho per column from rho per slot (batch mode: slot = spike id - 1 = col_key - 1)
@@ -565,10 +589,17 @@ hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qli
 }
 
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
-                             const double* dk2, float* rho_slot, float* dk_slot, int* knn32) {
+                             const double* dk2, float* rho_slot, float* dk_slot, int* knn32, int64_t* knn64) {
     if (nq <= 0) return hipSuccess;
     k_knn_finish<<<(unsigned)((nq + FIN_Q - 1) / FIN_Q), 256, 0, st>>>(p, nq, mode, knn_col, dk2, rho_slot, dk_slot,
-                                                                       knn32);
+                                                                       knn32, knn64);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_dists(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, float* out) {
+    const int64_t n = (int64_t)nq * p.k;
+    if (n <= 0) return hipSuccess;
+    k_knn_dists<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, nq, knn_col, out);
     return hipGetLastError();
 }
 

@@ -17,7 +17,10 @@ hipError_t launch_knn_exact(hipStream_t st, const DevPlan& p, const int* qlist, 
 hipError_t launch_knn_exact_dev(hipStream_t st, const DevPlan& p, const int* qlist, const int* nq_dev, int nq_max,
                                 int grid, int* knn_col, double* dk2);
 hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode, const int* knn_col,
-                             const double* dk2, float* rho_slot, float* dk_slot, int* knn32);
+                             const double* dk2, float* rho_slot, float* dk_slot, int* knn32,
+                             int64_t* knn64 = nullptr);
+// out[q * k + j] = float(sqrt(d2)) of the j-th listed neighbour of query q (canonical FP64 distance)
+hipError_t launch_knn_dists(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, float* out);
 hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, const float* rho_slot,
                            float* rho_col);
 hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col,

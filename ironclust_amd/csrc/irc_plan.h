@@ -32,6 +32,9 @@ hipError_t launch_plan(hipStream_t st, int S, int C, const DevPlan& p, const int
 // device counters of a call (read once after the call's last kernel): *dst = s32 ? *s32 : *s64
 hipError_t launch_stat_copy(hipStream_t st, const int* s32, const int64_t* s64, int64_t* dst);
 hipError_t launch_zero_i64(hipStream_t st, int64_t* p, int n);
+// *count += number of 32-bit words that differ between a and b (n_words each); *count starts at 0
+hipError_t launch_count_diff(hipStream_t st, const uint32_t* a, const uint32_t* b, int64_t n_words,
+                             unsigned long long* count);
 // largest candidate count of a block holding queries (atomicMax into *out, which starts at 0)
 hipError_t launch_max_ncand(hipStream_t st, int S, int C, const int* link_ptr, const int* link_idx, const int* run_len,
                             int* out);

@@ -205,6 +205,26 @@ hipError_t launch_stat_copy(hipStream_t st, const int* s32, const int64_t* s64, 
     return hipGetLastError();
 }
 
+namespace {
+__global__ void k_count_diff(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, int64_t n,
+                             unsigned long long* __restrict__ count) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += a[i] != b[i];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+}  // namespace
+
+hipError_t launch_count_diff(hipStream_t st, const uint32_t* a, const uint32_t* b, int64_t n_words,
+                             unsigned long long* count) {
+    if (n_words <= 0) return hipSuccess;
+    const int64_t blocks = (n_words + 255) / 256;
+    k_count_diff<<<(unsigned)(blocks < 4096 ? blocks : 4096), 256, 0, st>>>(a, b, n_words, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_zero_i64(hipStream_t st, int64_t* p, int n) {
     k_zero_i64<<<(n + 63) / 64, 64, 0, st>>>(p, n);
     return hipGetLastError();

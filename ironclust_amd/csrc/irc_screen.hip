@@ -886,11 +886,7 @@ __global__ void k_mfma_probe(float* out) {
 
 template <int PREC, int KB>
 hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
-#define IRC_SK(KPB)                                                                          \
-    do {                                                                                     \
-        if (a.hist) k_screen_knn<PREC, KB, KPB, true><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);   \
-        else k_screen_knn<PREC, KB, KPB, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg);         \
-    } while (0)
+#define IRC_SK(KPB) k_screen_knn<PREC, KB, KPB, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg)
     switch (a.kprime) {
         case 32: IRC_SK(32); break;
         case 40: IRC_SK(40); break;

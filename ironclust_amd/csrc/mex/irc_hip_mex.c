@@ -1,19 +1,23 @@
 /*
  * irc_hip_mex.c -- MATLAB MEX gateway over the C ABI of libirc_hip.so (include/irc_hip.h).
- * It holds no logic: it maps mxArrays to the ABI structs and errors to
- * mexErrMsgIdAndTxt('irc:hip', ...), which the caller's try/catch turns into its CPU
- * fallback (matlab/irc2.m:3643-3652, 3694-3702).  Build where MATLAB's mex.h exists:
+ * It holds no logic: it checks the mxArray sizes, maps them to the ABI structs and maps errors to
+ * mexErrMsgIdAndTxt('irc:hip', ...), which the caller's try/catch turns into its CPU fallback
+ * (matlab/irc2.m:3643-3652, 3694-3702).  Build where MATLAB's mex.h exists:
  *   mex -R2018a irc_hip_mex.c -I<repo>/include -L<repo>/ironclust_amd/_lib -lirc_hip
  *
  * Commands (1-based indices in and out, MATLAB orientation):
  *   [vrKnn_in, miKnn_in]    = irc_hip_mex('knn_drift', vl_in, mrFet_out, viDrift_out, mlDrift1, knn)
- *                              replaces search_knn_drift_ (matlab/irc2.m:3731-3789)
+ *                              replaces search_knn_drift_ (matlab/irc2.m:3731-3789); miKnn_in int32
+ *                              column indices, as the reference
  *   [vrDelta_in, viNneigh_in] = irc_hip_mex('delta_drift', vl_in, mrFet_out, vrRho_out, viDrift_out, mlDrift1)
  *                              replaces search_delta_drift_ (matlab/irc2.m:3794-3865)
  *   [vrRho, vrDelta, viNneigh, miKnn] = irc_hip_mex('rho_delta_page', mrFet, viSpk_col, viSite_ptr,
  *                              vnPrimary_site, viLim_drift, mlDrift, knn)
- *                              replaces rho_page_ + delta_page_ (matlab/irc2.m:3524-3607)
+ *                              replaces rho_page_ + delta_page_ (matlab/irc2.m:3524-3607); viNneigh and
+ *                              miKnn are int64 spike ids (irc2.m:3670, 3723), the type _knn_<site>.irc holds
  *   [vrKnn, miKnn]          = irc_hip_mex('knn_ab', mrFet2, mrFet1, knn)   replaces cuda_knn_ (irc2.m:4202)
+ *   [mrKnn, miKnn]          = irc_hip_mex('knn_ab_full', mrFet2, mrFet1, knn)   the cuda_knn_full form
+ *   [vrKnn, miKnn]          = irc_hip_mex('knn_ab_ib', mrFet2, viB, mrFet1, knn) the cuda_knn_ib form
  *   irc_hip_mex('reset')   destroys the persistent context
  * Optional trailing argument: precision (0 exact, 1 FP32 screen, 2 BF16 screen, 3 auto).
  */
@@ -46,8 +50,10 @@ static void need(int ok, const char* what) {
     if (!ok) mexErrMsgIdAndTxt("irc:hip:arg", "%s", what);
 }
 
+static mwSize numel(const mxArray* a) { return mxGetNumberOfElements(a); }
+
 static int32_t* to_i32(const mxArray* a) {
-    const mwSize n = mxGetNumberOfElements(a);
+    const mwSize n = numel(a);
     int32_t* v = (int32_t*)mxMalloc(sizeof(int32_t) * (n ? n : 1));
     const double* d = mxIsDouble(a) ? mxGetPr(a) : NULL;
     for (mwSize i = 0; i < n; ++i) {
@@ -61,17 +67,22 @@ static int32_t* to_i32(const mxArray* a) {
 }
 
 static int64_t* to_i64(const mxArray* a) {
-    const mwSize n = mxGetNumberOfElements(a);
+    const mwSize n = numel(a);
     int64_t* v = (int64_t*)mxMalloc(sizeof(int64_t) * (n ? n : 1));
-    int32_t* t = to_i32(a);
-    for (mwSize i = 0; i < n; ++i) v[i] = t[i];
-    if (mxIsInt64(a)) for (mwSize i = 0; i < n; ++i) v[i] = ((const int64_t*)mxGetData(a))[i];
-    mxFree(t);
+    if (mxIsInt64(a)) {
+        for (mwSize i = 0; i < n; ++i) v[i] = ((const int64_t*)mxGetData(a))[i];
+    } else if (mxIsDouble(a)) {
+        for (mwSize i = 0; i < n; ++i) v[i] = (int64_t)mxGetPr(a)[i];
+    } else {
+        int32_t* t = to_i32(a);
+        for (mwSize i = 0; i < n; ++i) v[i] = t[i];
+        mxFree(t);
+    }
     return v;
 }
 
 static uint8_t* to_u8(const mxArray* a) {
-    const mwSize n = mxGetNumberOfElements(a);
+    const mwSize n = numel(a);
     uint8_t* v = (uint8_t*)mxMalloc(n ? n : 1);
     if (mxIsLogical(a)) { const mxLogical* l = mxGetLogicals(a); for (mwSize i = 0; i < n; ++i) v[i] = l[i] ? 1 : 0; }
     else { int32_t* t = to_i32(a); for (mwSize i = 0; i < n; ++i) v[i] = t[i] != 0; mxFree(t); }
@@ -85,10 +96,11 @@ static void opts_from(irc_options* o, int nrhs, const mxArray* prhs[], int pos) 
 
 /* contiguous run marked by vl_in -> (in_begin, n_in) */
 static void in_range(const mxArray* vl, int64_t* beg, int64_t* n) {
-    const mwSize m = mxGetNumberOfElements(vl);
+    const mwSize m = numel(vl);
     const mxLogical* l = mxGetLogicals(vl);
     int64_t b = -1, e = -1;
     for (mwSize i = 0; i < m; ++i) if (l[i]) { if (b < 0) b = (int64_t)i; e = (int64_t)i; }
+    for (int64_t i = b; b >= 0 && i <= e; ++i) need(l[i] != 0, "vl_in must mark a contiguous run of columns");
     *beg = b < 0 ? 0 : b;
     *n = b < 0 ? 0 : e - b + 1;
 }
@@ -105,25 +117,31 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         const mxArray* F = prhs[2];
         const mxArray* drift = prhs[is_knn ? 3 : 4];
         const mxArray* link = prhs[is_knn ? 4 : 5];
+        const mwSize n_cols = mxGetN(F);
+        need(numel(prhs[1]) == n_cols, "numel(vl_in) must equal size(mrFet_out, 2)");
+        need(numel(drift) == n_cols, "numel(viDrift_out) must equal size(mrFet_out, 2)");
+        need(mxGetM(link) == mxGetN(link) && mxGetM(link) > 0, "mlDrift1 must be square");
         irc_site_problem pr;
         memset(&pr, 0, sizeof(pr));
         pr.D = (int32_t)mxGetM(F);
-        pr.n_cols = (int64_t)mxGetN(F);
+        pr.n_cols = (int64_t)n_cols;
         pr.F = (const float*)mxGetData(F);
+        in_range(prhs[1], &pr.in_begin, &pr.n_in);
         int32_t* dr = to_i32(drift);
         uint8_t* lk = to_u8(link);
         pr.drift_of_col = dr;
         pr.n_drift = (int32_t)mxGetM(link);
         pr.link = lk;
-        in_range(prhs[1], &pr.in_begin, &pr.n_in);
         if (is_knn) {
             pr.k = (int32_t)mxGetScalar(prhs[5]);
+            need(pr.k >= 1, "knn must be positive");
             opts_from(&o, nrhs, prhs, 6);
             plhs[0] = mxCreateNumericMatrix((mwSize)pr.n_in, 1, mxSINGLE_CLASS, mxREAL);
             plhs[1] = mxCreateNumericMatrix((mwSize)pr.k, (mwSize)pr.n_in, mxINT32_CLASS, mxREAL);
             check(irc_knn_drift(ctx(), &pr, &o, (float*)mxGetData(plhs[0]), (int32_t*)mxGetData(plhs[1])));
         } else {
             need(mxIsSingle(prhs[3]), "vrRho_out must be single");
+            need(numel(prhs[3]) == n_cols, "numel(vrRho_out) must equal size(mrFet_out, 2)");
             pr.k = 1;
             opts_from(&o, nrhs, prhs, 6);
             plhs[0] = mxCreateNumericMatrix((mwSize)pr.n_in, 1, mxSINGLE_CLASS, mxREAL);
@@ -137,26 +155,36 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     }
     if (!strcmp(cmd, "rho_delta_page")) {
         need(nrhs >= 8 && mxIsSingle(prhs[1]), "bad arguments");
+        const mwSize n_cols = mxGetN(prhs[1]);
+        const mwSize S = numel(prhs[4]);
+        need(S >= 1, "vnPrimary_site is empty");
+        need(numel(prhs[2]) == n_cols, "numel(viSpk_col) must equal size(mrFet, 2)");
+        need(numel(prhs[3]) == S + 1, "numel(viSite_ptr) must equal numel(vnPrimary_site) + 1");
+        need(numel(prhs[5]) >= 2, "viLim_drift needs at least two entries");
+        const mwSize C = numel(prhs[5]) - 1;
+        need(mxGetM(prhs[6]) == C && mxGetN(prhs[6]) == C, "mlDrift must be n_drift x n_drift");
         irc_batch b;
         memset(&b, 0, sizeof(b));
         b.D = (int32_t)mxGetM(prhs[1]);
-        b.n_cols = (int64_t)mxGetN(prhs[1]);
+        b.n_cols = (int64_t)n_cols;
         b.F = (const float*)mxGetData(prhs[1]);
-        int32_t* spk = to_i32(prhs[2]);
         int64_t* sp = to_i64(prhs[3]);
+        need(sp[S] == (int64_t)n_cols, "viSite_ptr(end) must equal size(mrFet, 2)");
+        int32_t* spk = to_i32(prhs[2]);
         int64_t* np = to_i64(prhs[4]);
         int64_t* lim = to_i64(prhs[5]);
         uint8_t* lk = to_u8(prhs[6]);
         b.col_spike = spk;
         b.site_ptr = sp;
         b.site_n_primary = np;
-        b.n_sites = (int32_t)mxGetNumberOfElements(prhs[4]);
+        b.n_sites = (int32_t)S;
         b.lim_drift = lim;
-        b.n_drift = (int32_t)mxGetNumberOfElements(prhs[5]) - 1;
+        b.n_drift = (int32_t)C;
         b.link = lk;
         b.k = (int32_t)mxGetScalar(prhs[7]);
+        need(b.k >= 1, "knn must be positive");
         b.n_spikes = 0;
-        for (int32_t s = 0; s < b.n_sites; ++s) b.n_spikes += np[s];
+        for (mwSize s = 0; s < S; ++s) b.n_spikes += np[s];
         opts_from(&o, nrhs, prhs, 8);
         plhs[0] = mxCreateNumericMatrix((mwSize)b.n_spikes, 1, mxSINGLE_CLASS, mxREAL);
         plhs[1] = mxCreateNumericMatrix((mwSize)b.n_spikes, 1, mxSINGLE_CLASS, mxREAL);
@@ -167,23 +195,41 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         out.delta = (float*)mxGetData(plhs[1]);
         out.nneigh = (int64_t*)mxGetData(plhs[2]);
         if (nlhs > 3) {
-            plhs[3] = mxCreateNumericMatrix((mwSize)b.k, (mwSize)b.n_spikes, mxINT32_CLASS, mxREAL);
-            out.knn = (int32_t*)mxGetData(plhs[3]);
+            plhs[3] = mxCreateNumericMatrix((mwSize)b.k, (mwSize)b.n_spikes, mxINT64_CLASS, mxREAL);
+            out.knn = (int64_t*)mxGetData(plhs[3]);
         }
         check(irc_rho_delta_batch(ctx(), &b, &o, &out));
         mxFree(spk); mxFree(sp); mxFree(np); mxFree(lim); mxFree(lk);
         return;
     }
-    if (!strcmp(cmd, "knn_ab")) {
-        need(nrhs >= 4 && mxIsSingle(prhs[1]) && mxIsSingle(prhs[2]), "bad arguments");
+    if (!strcmp(cmd, "knn_ab") || !strcmp(cmd, "knn_ab_full") || !strcmp(cmd, "knn_ab_ib")) {
+        const int ib = !strcmp(cmd, "knn_ab_ib"), full = !strcmp(cmd, "knn_ab_full");
+        const int ia = ib ? 3 : 2;   /* position of mrFet1 */
+        need(nrhs >= ia + 2 && mxIsSingle(prhs[1]) && mxIsSingle(prhs[ia]), "bad arguments");
+        need(mxGetM(prhs[1]) == mxGetM(prhs[ia]), "mrFet2 and mrFet1 must have the same number of rows");
         const int32_t D = (int32_t)mxGetM(prhs[1]);
-        const int64_t nB = (int64_t)mxGetN(prhs[1]), nA = (int64_t)mxGetN(prhs[2]);
-        int32_t k = (int32_t)mxGetScalar(prhs[3]);
-        if (k > nB) k = (int32_t)nB;
-        plhs[0] = mxCreateNumericMatrix((mwSize)nA, 1, mxSINGLE_CLASS, mxREAL);
+        const int64_t nB = (int64_t)mxGetN(prhs[1]), nA = (int64_t)mxGetN(prhs[ia]);
+        const int64_t nC = ib ? (int64_t)numel(prhs[2]) : nB;   /* candidates */
+        need(nB >= 1 && nC >= 1, "no candidates");
+        int32_t k = (int32_t)mxGetScalar(prhs[ia + 1]);
+        need(k >= 1, "knn must be positive");
+        if (k > nC) k = (int32_t)nC;
+        plhs[0] = mxCreateNumericMatrix(full ? (mwSize)k : (mwSize)nA, full ? (mwSize)nA : 1, mxSINGLE_CLASS, mxREAL);
         plhs[1] = mxCreateNumericMatrix((mwSize)k, (mwSize)nA, mxINT32_CLASS, mxREAL);
-        check(irc_knn_ab(ctx(), D, nB, (const float*)mxGetData(prhs[1]), nA, (const float*)mxGetData(prhs[2]), k,
-                         (float*)mxGetData(plhs[0]), (int32_t*)mxGetData(plhs[1])));
+        const float* B = (const float*)mxGetData(prhs[1]);
+        const float* A = (const float*)mxGetData(prhs[ia]);
+        float* K = (float*)mxGetData(plhs[0]);
+        int32_t* I = (int32_t*)mxGetData(plhs[1]);
+        if (ib) {
+            int32_t* vb = to_i32(prhs[2]);
+            const irc_status st = irc_knn_ab_ib(ctx(), D, nB, B, nC, vb, nA, A, k, K, I);
+            mxFree(vb);
+            check(st);
+        } else if (full) {
+            check(irc_knn_ab_full(ctx(), D, nB, B, nA, A, k, K, I));
+        } else {
+            check(irc_knn_ab(ctx(), D, nB, B, nA, A, k, K, I));
+        }
         return;
     }
     mexErrMsgIdAndTxt("irc:hip:arg", "unknown command '%s'", cmd);

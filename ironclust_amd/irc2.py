@@ -11,6 +11,9 @@ are D x n (one column per spike), indices returned are 1-based.
                       delta_paged_site_ folded into one batched call)
   plan_sort_page_     matlab/irc2.m:3914-3946
   calc_drift_         matlab/irc2.m:6373-6447 (drift linkage that feeds the path)
+  sort_               matlab/irc2.m:3251-3274 (calc_drift_ -> sort_page_ -> rho/max(rho) -> ordrho)
+  sort_cache_         matlab/irc2.m:359-383 (the save/load of its result: irc_files.save_sort_)
+  cuda_knn_full_, cuda_knn_ib_  the cuda_knn_full.cu / cuda_knn_ib.cu forms
   prepare_page_site_  matlab/irc2.m:3869-3910 (site_layout builds the same column grouping
                       for every site at once)
 
@@ -92,6 +95,29 @@ def cuda_knn_(mrFet2, mrFet1, P=None):
     return K, 1, np.ascontiguousarray(I.T)
 
 
+def cuda_knn_full_(mrFet2, mrFet1, P=None):
+    """[mrKnn, miKnn] of the cuda_knn_full kernel (cuda_knn_full.cu:27-100): every one of the knn
+    nearest columns of mrFet2 for each column of mrFet1, distances (knn x n1) and 1-based indices."""
+    P = P or {}
+    B = np.ascontiguousarray(np.asarray(mrFet2, dtype=np.float32).T)
+    A = np.ascontiguousarray(np.asarray(mrFet1, dtype=np.float32).T)
+    knn = min(int(P.get("knn", 30)), B.shape[0])
+    K, I = get_ctx(P.get("device", 0)).knn_ab_full(B, A, knn)
+    return np.ascontiguousarray(K.T), np.ascontiguousarray(I.T)
+
+
+def cuda_knn_ib_(mrFet2, viB, mrFet1, P=None):
+    """[vrKnn, miKnn] of the cuda_knn_ib kernel (cuda_knn_ib.cu:27-97): the candidates are the
+    columns viB (1-based) of mrFet2; miKnn holds columns of mrFet2."""
+    P = P or {}
+    B = np.ascontiguousarray(np.asarray(mrFet2, dtype=np.float32).T)
+    A = np.ascontiguousarray(np.asarray(mrFet1, dtype=np.float32).T)
+    ib = np.asarray(viB).ravel()
+    knn = min(int(P.get("knn", 30)), ib.size)
+    K, I = get_ctx(P.get("device", 0)).knn_ab_ib(B, ib, A, knn)
+    return K, np.ascontiguousarray(I.T)
+
+
 def calc_drift_(vrAmp_spk, mrPos_spk, P=None):
     """S_drift = calc_drift_(S0, P) (matlab/irc2.m:6373-6447) on the spikes' amplitudes and
     positions: returns dict(nTime_drift, mlDrift (C x C logical), viLim_drift (1-based),
@@ -162,6 +188,59 @@ def sort_page_(layout, P=None, want_knn=True):
     if want_knn:
         out.append(np.ascontiguousarray(r["knn"].T))
     return tuple(out)
+
+
+def ordrho_(vrRho):
+    """[~, ordrho] = sort(vrRho, 'descend') (matlab/irc2.m:3269): 1-based positions (double), equal
+    values in ascending position (MATLAB's sort is stable), NaN first (a descending sort places NaN
+    before every number)."""
+    r = np.asarray(vrRho).ravel()
+    idx = np.arange(r.size)
+    nan = np.isnan(r)
+    rest = idx[~nan]
+    o = np.concatenate([idx[nan], rest[np.argsort(-r[~nan], kind="stable")]])
+    return (o + 1).astype(np.float64)
+
+
+def sort_(S0, P=None):
+    """S_clu = sort_(S0, P) (matlab/irc2.m:3251-3274): drift linkage, then every site and drift chunk
+    in one batch call, rho divided by its maximum (3268) and ordrho (3269).
+    S0 holds the drift inputs vrAmp_spk, mrPos_spk and the features, either
+      * in memory: F1 (N x D rows), viSite_spk, and optionally F2, viSite2_spk, or
+      * as detect files: vcFile_prm (the prefix the _fet_ files were written under),
+        ccviSpk_site_load, ccviSpk_site2_load (or None), dimm_fet, and mlPc (get_mlPc_).
+    P: calc_drift_'s keys (nTime_batch, nTime_drift, nQuantile_drift, nTime_max_drift, viSite2Chan or
+    nSites), knn, precision; P['vcFile_prm'] set: the per-site _knn_<site>.irc files are written as
+    rho_paged_site_ does (matlab/irc2.m:3670-3679).  Returns S_clu (rho, delta, ordrho, nneigh, P,
+    miKnn (k x N int64 when P['fKeep_knn'], else empty, as the reference), S_drift, nFeatures)."""
+    from . import irc_files
+    P = dict(P or {})
+    knn = int(P.get("knn", 30))
+    S_drift = calc_drift_(S0["vrAmp_spk"], S0["mrPos_spk"], P)
+    lim, link = S_drift["viLim_drift"], S_drift["mlDrift"]
+    N = int(np.asarray(S0["vrAmp_spk"]).size)
+    if "ccviSpk_site_load" in S0:
+        layout = irc_files.layout_from_fet_files(S0["vcFile_prm"], S0["ccviSpk_site_load"],
+                                                 S0.get("ccviSpk_site2_load"), S0["dimm_fet"], S0["mlPc"], lim,
+                                                 link, N, knn)
+    else:
+        import torch
+        site1 = torch.as_tensor(np.asarray(S0["viSite_spk"], dtype=np.int32))
+        has2 = S0.get("viSite2_spk") is not None
+        ds = dict(n_spikes=N, n_sites=int(S0.get("nSites", int(site1.max()))), k=knn,
+                  F1=torch.as_tensor(np.asarray(S0["F1"], dtype=np.float32)), site1=site1,
+                  F2=torch.as_tensor(np.asarray(S0["F2"], dtype=np.float32)) if has2 else None,
+                  site2=torch.as_tensor(np.asarray(S0["viSite2_spk"], dtype=np.int32)) if has2 else None,
+                  lim=lim, link=np.asarray(link, dtype=np.uint8))
+        layout = site_layout(ds)
+    vrRho, vrDelta, viNneigh, miKnn = sort_page_(layout, dict(P, knn=knn))
+    if P.get("vcFile_prm"):
+        irc_files.save_knn_by_site(P["vcFile_prm"], {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v)
+                                                      for kk, v in layout.items()}, miKnn)
+    vrRho = (vrRho / vrRho.max()).astype(np.float32)
+    return dict(rho=vrRho, delta=vrDelta, ordrho=ordrho_(vrRho), nneigh=viNneigh, P=P,
+                miKnn=miKnn if P.get("fKeep_knn") else np.zeros((0, 0), np.int64), S_drift=S_drift,
+                nFeatures=int(layout["F"].shape[1]))
 
 
 def plan_sort_page_(nTime_drift, viLim_drift, nTime_max_drift=256):

@@ -9,8 +9,13 @@ HIP path and read the kNN back (host side; plain numpy file I/O).
         indices plus the count of the spikes of earlier loads (detect_merge_, matlab/irc2.m:4536-4577)
   load_fet_site_    one site's features over every load, masked by mlPc (matlab/irc2.m:4039-4135)
   get_mlPc_         the PC x site feature mask (matlab/irc2.m:3278-3312)
-  _knn_<site>.irc   raw column-major int32 knn x n, appended page by page
-        (save_miKnn_site_, matlab/irc2.m:3355-3369)
+  _knn_<site>.irc   raw column-major int64 knn x n global spike ids, appended page by page
+        (save_miKnn_site_, matlab/irc2.m:3355-3369, of miKnn_in = int64(...), 3670; read back as
+        'int64' by load_miKnn_site_, 3395)
+
+  sort.irc + sort_irc.mat   the sort stage's cache (sort_cache_, matlab/irc2.m:359-383): rho, delta,
+        nneigh and ordrho written back to back by struct_save_bin_ (matlab/irc2.m:1733-1753), their
+        class and size in the S_var struct saved beside them; struct_load_bin_ (1775-1805) reads it
 
 `layout_from_fet_files` assembles the site-grouped column layout that irc_rho_delta_batch takes
 (primary columns of a site from _fet_, then its secondary columns from _fet2_), which is exactly
@@ -132,15 +137,22 @@ def load_fet_site_(prefix, iFet, iSite, ccviSpk_site_load, dimm_fet, mlPc=None):
 
 
 def save_miKnn_site_(prefix, iSite, miKnn1, fAppend=False):
-    """Write (or append) a knn x n int32 block to _knn_<iSite>.irc (matlab/irc2.m:3355-3369)."""
+    """Write (or append) a knn x n int64 block to _knn_<iSite>.irc (matlab/irc2.m:3355-3369): the
+    reference's miKnn_in is int64 (matlab/irc2.m:3670), and fwrite writes its class."""
     path = f"{prefix}_knn_{iSite}.irc"
-    write_bin_(path, np.asarray(miKnn1, dtype=np.int32), append=fAppend)
+    a = np.asarray(miKnn1)
+    if a.dtype != np.int64:
+        if a.size and (a.min() < np.iinfo(np.int64).min or not np.issubdtype(a.dtype, np.integer)):
+            raise TypeError("miKnn must hold integer spike ids")
+        a = a.astype(np.int64)
+    write_bin_(path, a, append=fAppend)
     return path
 
 
 def load_miKnn_site_(prefix, iSite, knn):
-    """Read _knn_<iSite>.irc back as knn x n int32."""
-    a = np.fromfile(f"{prefix}_knn_{iSite}.irc", dtype=np.int32)
+    """Read _knn_<iSite>.irc back as knn x n int64 (load_bin_(..., 'int64', [knn, n]),
+    matlab/irc2.m:3395)."""
+    a = np.fromfile(f"{prefix}_knn_{iSite}.irc", dtype=np.int64)
     return a.reshape((knn, -1), order="F")
 
 
@@ -183,3 +195,90 @@ def save_knn_by_site(prefix, layout, knn, fAppend=False):
         ids = cs[sp[s]:sp[s] + npr[s]].astype(np.int64)
         paths.append(save_miKnn_site_(prefix, s + 1, K[:, ids - 1], fAppend))
     return paths
+
+
+# ---- the sort stage's cache: struct_save_bin_ / struct_load_bin_ ----------------------------------
+
+_MATLAB_CLASS = {np.dtype(np.float32): "single", np.dtype(np.float64): "double", np.dtype(np.int64): "int64",
+                 np.dtype(np.int32): "int32", np.dtype(np.uint32): "uint32", np.dtype(np.int16): "int16",
+                 np.dtype(np.uint8): "uint8", np.dtype(np.bool_): "logical"}
+_NUMPY_TYPE = {v: k for k, v in _MATLAB_CLASS.items()}
+
+
+def _matlab_dims(a):
+    """size() of the array as MATLAB reports it: vectors are columns (n x 1)."""
+    return (a.shape[0], 1) if a.ndim == 1 else tuple(a.shape)
+
+
+def struct_save_bin_(S, vcFile, csName_var):
+    """[S0, S_var] = struct_save_bin_(S0, vcFile, csName_var) (matlab/irc2.m:1733-1753): the named
+    fields are written back to back (fwrite of each class, column-major) into vcFile and cleared in the
+    returned copy of S; S_var records vcFile, the names, classes (csType_var) and sizes (cDimm_var)."""
+    S0 = dict(S)
+    S_var = {"vcFile": vcFile, "csName_var": list(csName_var), "csType_var": [], "cDimm_var": []}
+    with open(vcFile, "wb") as fh:
+        for name in csName_var:
+            a = np.asarray(S0[name])
+            S0[name] = np.zeros((0, 0))
+            if a.dtype not in _MATLAB_CLASS:
+                raise TypeError(f"field {name}: no MATLAB class for {a.dtype}")
+            write_bin_(fh, a)
+            S_var["csType_var"].append(_MATLAB_CLASS[a.dtype])
+            S_var["cDimm_var"].append(_matlab_dims(a))
+    return S0, S_var
+
+
+def struct_load_bin_(S_var, S=None, csVar_load=()):
+    """S = struct_load_bin_(S_var, S, csVar_load) (matlab/irc2.m:1775-1805): each variable is read in
+    turn with its class and size (fields not in csVar_load are skipped when it is given)."""
+    S = dict(S or {})
+    with open(S_var["vcFile"], "rb") as fh:
+        for name, typ, dimm in zip(S_var["csName_var"], S_var["csType_var"], S_var["cDimm_var"]):
+            dt = _NUMPY_TYPE[str(typ)]
+            dimm = tuple(int(x) for x in np.ravel(dimm))
+            n = int(np.prod(dimm))
+            if csVar_load and not any(v in name for v in csVar_load):
+                fh.seek(n * dt.itemsize, 1)
+                continue
+            a = np.fromfile(fh, dtype=dt, count=n)
+            if a.size != n:
+                raise IOError(f"{S_var['vcFile']}: {name} is truncated")
+            S[name] = a.reshape(dimm, order="F")
+    return S
+
+
+def save_sort_(S_clu, vcFile_sort):
+    """The save half of sort_cache_ (matlab/irc2.m:378-381): rho, delta, nneigh, ordrho go to
+    <base>.irc through struct_save_bin_, everything else plus S_var to vcFile_sort (a MAT file;
+    written as MAT v5, which MATLAB's load reads like the reference's v7.3).  vcFile_sort names the
+    .mat file ('.../sort_irc.mat').  Returns (path of the .irc blob, S_var)."""
+    from scipy.io import savemat
+    vcFile_bin = vcFile_sort.replace("_irc.mat", ".irc")
+    S_, S_var = struct_save_bin_(S_clu, vcFile_bin, ["rho", "delta", "nneigh", "ordrho"])
+    def storable(v):   # what savemat writes as a MATLAB value (dicts become structs)
+        if isinstance(v, dict):
+            return all(storable(x) for x in v.values())
+        return isinstance(v, (int, float, str, bool, np.integer, np.floating)) or (
+            isinstance(v, np.ndarray) and v.dtype != object)
+
+    mat = {k: ({kk: vv for kk, vv in v.items() if storable(vv)} if isinstance(v, dict) else v)
+           for k, v in S_.items() if k not in S_var["csName_var"] and (isinstance(v, dict) or storable(v))}
+    mat["S_var"] = {"vcFile": vcFile_bin,
+                    "csName_var": np.array(S_var["csName_var"], dtype=object),
+                    "csType_var": np.array(S_var["csType_var"], dtype=object),
+                    "cDimm_var": np.array([np.array(d, dtype=np.float64) for d in S_var["cDimm_var"]], dtype=object)}
+    savemat(vcFile_sort, mat)
+    return vcFile_bin, S_var
+
+
+def load_sort_(vcFile_sort):
+    """The load half of sort_cache_ (matlab/irc2.m:366-368): S_clu = load(vcFile_sort), then
+    struct_load_bin_(S_clu.S_var, S_clu)."""
+    from scipy.io import loadmat
+    m = loadmat(vcFile_sort, squeeze_me=True, simplify_cells=True)
+    sv = m.pop("S_var")
+    S_var = {"vcFile": str(sv["vcFile"]), "csName_var": [str(x) for x in np.ravel(sv["csName_var"])],
+             "csType_var": [str(x) for x in np.ravel(sv["csType_var"])],
+             "cDimm_var": [np.ravel(x).astype(np.int64) for x in sv["cDimm_var"]]}
+    S = {k: v for k, v in m.items() if not k.startswith("__")}
+    return struct_load_bin_(S_var, S)

@@ -14,13 +14,13 @@ LIB_PATH = os.path.join(_PKG, "_lib", "libirc_hip.so")
 IRC_OK, IRC_E_ARG, IRC_E_NOMEM, IRC_E_HIP, IRC_E_UNSUPPORTED, IRC_E_FORCED = range(6)
 PREC_EXACT, PREC_FP32, PREC_BF16_SCREEN, PREC_AUTO = range(4)
 MEM_HOST, MEM_DEVICE = 0, 1
-FLAG_HIST, FLAG_GRAPH = 2, 4          # irc_options.flags
+FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_SITE_LAUNCHES = 4, 8, 16          # irc_options.flags
 
 EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
-    "irc_knn_ab", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
-    "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts",
+    "irc_knn_ab", "irc_knn_ab_full", "irc_knn_ab_ib", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
+    "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts", "irc_autotune",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -57,7 +57,7 @@ class DriftProblem(ctypes.Structure):
 
 
 class Outputs(ctypes.Structure):
-    _fields_ = [("rho", _c_f32p), ("delta", _c_f32p), ("nneigh", _c_i64p), ("knn", _c_i32p), ("dk", _c_f32p)]
+    _fields_ = [("rho", _c_f32p), ("delta", _c_f32p), ("nneigh", _c_i64p), ("knn", _c_i64p), ("dk", _c_f32p)]
 
 
 class Stats(ctypes.Structure):
@@ -67,6 +67,13 @@ class Stats(ctypes.Structure):
                 ("ms_screen", ctypes.c_double), ("precision_used", ctypes.c_int32),
                 ("ms_rerank", ctypes.c_double), ("n_rerank_overflow", ctypes.c_int64),
                 ("ms_graph", ctypes.c_double), ("n_rerank_second", ctypes.c_int64)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("n_trials", ctypes.c_int32), ("best_trial", ctypes.c_int32), ("ms_base", ctypes.c_double),
+                ("ms_best", ctypes.c_double), ("k_over", ctypes.c_int32 * 16), ("waves_per_cu", ctypes.c_int32 * 16),
+                ("tile", ctypes.c_int32 * 16), ("large_ncand", ctypes.c_int32 * 16), ("ms", ctypes.c_double * 16),
+                ("identical", ctypes.c_int32 * 16)]
 
 
 class IrcError(RuntimeError):
@@ -108,6 +115,9 @@ def load(path=None):
     lib.irc_rho_delta_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
                                         ctypes.POINTER(Outputs)]
     lib.irc_rho_delta_batch.restype = ctypes.c_int
+    lib.irc_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
+                                 ctypes.POINTER(Outputs), ctypes.POINTER(Options), ctypes.POINTER(TuneReport)]
+    lib.irc_autotune.restype = ctypes.c_int
     lib.irc_get_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.irc_get_stats.restype = ctypes.c_int
     lib.irc_selftest_mfma.argtypes = [ctypes.c_void_p, _c_f32p]
@@ -115,6 +125,12 @@ def load(path=None):
     lib.irc_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_f32p,
                                ctypes.c_int32, _c_f32p, _c_i32p]
     lib.irc_knn_ab.restype = ctypes.c_int
+    lib.irc_knn_ab_full.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_f32p,
+                                    ctypes.c_int32, _c_f32p, _c_i32p]
+    lib.irc_knn_ab_full.restype = ctypes.c_int
+    lib.irc_knn_ab_ib.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_i32p,
+                                  ctypes.c_int64, _c_f32p, ctypes.c_int32, _c_f32p, _c_i32p]
+    lib.irc_knn_ab_ib.restype = ctypes.c_int
     lib.irc_delta_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, _c_f32p,
                                      ctypes.c_int64, _c_f32p, _c_f32p, _c_f32p, _c_i32p]
     lib.irc_delta_knn_ab.restype = ctypes.c_int
@@ -129,7 +145,7 @@ def load(path=None):
     lib.irc_calc_drift.restype = ctypes.c_int
     lib.irc_assign_chain.argtypes = [ctypes.c_void_p, ctypes.c_int64, _c_i64p, _c_i32p, ctypes.POINTER(Options)]
     lib.irc_assign_chain.restype = ctypes.c_int
-    lib.irc_isolation_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _c_i32p, _c_i32p,
+    lib.irc_isolation_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, _c_i64p, _c_i32p,
                                          ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(Options)]
     lib.irc_isolation_counts.restype = ctypes.c_int
     _lib = lib
@@ -259,9 +275,9 @@ class Context:
         rho = np.zeros(N, np.float32)
         delta = np.zeros(N, np.float32)
         nn = np.zeros(N, np.int64)
-        knn = np.zeros((N, int(k)), np.int32) if want_knn else None
+        knn = np.zeros((N, int(k)), np.int64) if want_knn else None
         dk = np.zeros(N, np.float32) if want_dk else None
-        out = Outputs(_p(rho, _c_f32p), _p(delta, _c_f32p), _p(nn, _c_i64p), _p(knn, _c_i32p), _p(dk, _c_f32p))
+        out = Outputs(_p(rho, _c_f32p), _p(delta, _c_f32p), _p(nn, _c_i64p), _p(knn, _c_i64p), _p(dk, _c_f32p))
         o = opts or make_options()
         o.mem = MEM_HOST
         self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(out)))
@@ -272,11 +288,7 @@ class Context:
             res["dk"] = dk
         return res
 
-    def rho_delta_batch_device(self, dev, out, opts=None):
-        """Device-resident form.  dev: dict of torch tensors on the context's device
-        (F [n_cols, D] float32, col_spike int32) plus host arrays site_ptr, site_n_primary,
-        lim_drift, link, n_spikes, k.  out: dict of torch device tensors rho, delta, nneigh
-        (int64), optional knn (int64 [N, k]) and dk."""
+    def _device_structs(self, dev, out, opts):
         site_ptr = np.ascontiguousarray(dev["site_ptr"], dtype=np.int64)
         site_np = np.ascontiguousarray(dev["site_n_primary"], dtype=np.int64)
         lim = np.ascontiguousarray(dev["lim_drift"], dtype=np.int64)
@@ -291,7 +303,32 @@ class Context:
         o = opts or make_options()
         o.mem = MEM_DEVICE
         outs = Outputs(_dptr(out["rho"], _c_f32p), _dptr(out["delta"], _c_f32p), _dptr(out["nneigh"], _c_i64p),
-                       _dptr(out.get("knn"), _c_i32p), _dptr(out.get("dk"), _c_f32p))
+                       _dptr(out.get("knn"), _c_i64p), _dptr(out.get("dk"), _c_f32p))
+        return b, o, outs, (site_ptr, site_np, lim, lk)
+
+    def autotune_device(self, dev, out, opts=None):
+        """irc_autotune on device-resident data (as rho_delta_batch_device): returns (options dict of
+        the chosen k_over / waves_per_cu / tile / large_ncand, report dict)."""
+        b, o, outs, keep = self._device_structs(dev, out, opts)
+        best = Options()
+        rep = TuneReport()
+        self._check(self.lib.irc_autotune(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(outs),
+                                          ctypes.byref(best), ctypes.byref(rep)))
+        n = rep.n_trials
+        trials = [{"k_over": rep.k_over[i], "waves_per_cu": rep.waves_per_cu[i], "tile": rep.tile[i],
+                   "large_ncand": rep.large_ncand[i], "ms": rep.ms[i], "identical": bool(rep.identical[i])}
+                  for i in range(n)]
+        choice = {"k_over": best.k_over, "waves_per_cu": best.waves_per_cu, "tile": best.tile,
+                  "large_ncand": best.large_ncand}
+        return choice, {"ms_base": rep.ms_base, "ms_best": rep.ms_best, "best_trial": rep.best_trial,
+                        "trials": trials}
+
+    def rho_delta_batch_device(self, dev, out, opts=None):
+        """Device-resident form.  dev: dict of torch tensors on the context's device
+        (F [n_cols, D] float32, col_spike int32) plus host arrays site_ptr, site_n_primary,
+        lim_drift, link, n_spikes, k.  out: dict of torch device tensors rho, delta, nneigh
+        (int64), optional knn (int64 [N, k]: row = spike, the k ids nearest first) and dk."""
+        b, o, outs, keep = self._device_structs(dev, out, opts)
         self._check(self.lib.irc_rho_delta_batch(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(outs)))
 
     # ---- legacy A/B and dc forms ------------------------------------------------------
@@ -302,6 +339,27 @@ class Context:
         I = np.zeros((A.shape[0], int(k)), np.int32) if want_index else None
         self._check(self.lib.irc_knn_ab(self.h, B.shape[1], B.shape[0], _p(B, _c_f32p), A.shape[0],
                                         _p(A, _c_f32p), int(k), _p(K, _c_f32p), _p(I, _c_i32p)))
+        return K, I
+
+    def knn_ab_full(self, B, A, k):
+        """cuda_knn_full form: (K (nA, k) every neighbour's distance, I (nA, k) 1-based into B)."""
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        K = np.zeros((A.shape[0], int(k)), np.float32)
+        I = np.zeros((A.shape[0], int(k)), np.int32)
+        self._check(self.lib.irc_knn_ab_full(self.h, B.shape[1], B.shape[0], _p(B, _c_f32p), A.shape[0],
+                                             _p(A, _c_f32p), int(k), _p(K, _c_f32p), _p(I, _c_i32p)))
+        return K, I
+
+    def knn_ab_ib(self, B, IB, A, k, want_index=True):
+        """cuda_knn_ib form: candidates are the 1-based columns IB of B; (K (nA,), I (nA, k) B columns)."""
+        B = np.ascontiguousarray(B, dtype=np.float32)
+        A = np.ascontiguousarray(A, dtype=np.float32)
+        ib = np.ascontiguousarray(IB, dtype=np.int32)
+        K = np.zeros(A.shape[0], np.float32)
+        I = np.zeros((A.shape[0], int(k)), np.int32) if want_index else None
+        self._check(self.lib.irc_knn_ab_ib(self.h, B.shape[1], B.shape[0], _p(B, _c_f32p), ib.size, _p(ib, _c_i32p),
+                                           A.shape[0], _p(A, _c_f32p), int(k), _p(K, _c_f32p), _p(I, _c_i32p)))
         return K, I
 
     def delta_knn_ab(self, B, RB, A, RA):
@@ -367,13 +425,13 @@ class Context:
 
     def isolation_counts(self, knn, clu, n_clu):
         """Neighbour-cluster counts (n_clu, n_clu + 1) uint64: row c-1, column d = kNN entries of
-        cluster c's spikes that lie in cluster d (0: none).  knn: k x n 1-based (host)."""
-        K = np.asfortranarray(np.asarray(knn, dtype=np.int32))
+        cluster c's spikes that lie in cluster d (0: none).  knn: k x n 1-based spike ids (host)."""
+        K = np.asfortranarray(np.asarray(knn, dtype=np.int64))
         k, n = K.shape
         flat = np.ascontiguousarray(K.reshape(-1, order="F"))
         cl = np.ascontiguousarray(clu, dtype=np.int32)
         out = np.zeros((int(n_clu), int(n_clu) + 1), np.uint64)
-        self._check(self.lib.irc_isolation_counts(self.h, n, k, _p(flat, _c_i32p), _p(cl, _c_i32p), int(n_clu),
+        self._check(self.lib.irc_isolation_counts(self.h, n, k, _p(flat, _c_i64p), _p(cl, _c_i32p), int(n_clu),
                                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), None))
         return out
 

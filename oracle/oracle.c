@@ -34,6 +34,8 @@
  *   orc_rho_drift_dc    jrc3_cuda_rho_drift   matlab/irc_cuda_rho_drift.cu:28-108
  *   orc_delta_drift_dc  jrc3_cuda_delta_drift matlab/irc_cuda_delta_drift.cu:28-132
  *   orc_knn_ab          exact form of cuda_knn_index / knn_cpu_  matlab/cuda_knn_index.cu:27-111
+ *   orc_knn_ab_list     exact forms of cuda_knn_full / cuda_knn_ib matlab/cuda_knn_full.cu:27-100,
+ *                                                                 matlab/cuda_knn_ib.cu:27-97
  *   orc_delta_knn_ab    cuda_delta_knn / delta_knn_cpu_         matlab/cuda_delta_knn.cu:19-88
  *
  * Build with -ffp-contract=off: every fused multiply-add here is explicit.
@@ -147,10 +149,13 @@ int orc_sort_batch(int64_t N, int D, int S, int C, int k,
 #pragma omp parallel
     {
         orc_cand* buf = NULL; int64_t cap = 0;
-#pragma omp for schedule(dynamic, 1)
-        for (int s = 0; s < S; ++s) {
-            for (int64_t qi = 0; qi < g1[s].n; ++qi) {
-                int64_t a = g1[s].ids[qi];
+        /* every spike is a query of its primary site; queries are independent, so the loop runs over
+           spikes (a few large sites still use every thread) */
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t a = 0; a < N; ++a) {
+            {
+                const int s = site1[a] - 1;
+                if (s < 0 || s >= S) continue;
                 int c = chunk_of[a];
                 int64_t n = 0;
                 for (int li = lptr[c]; li < lptr[c + 1]; ++li) {
@@ -185,10 +190,11 @@ int orc_sort_batch(int64_t N, int D, int S, int C, int k,
     }
     /* pass 2: delta, nneigh */
     const float sent = (float)sqrt(ORC_SINGLE_INF);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int s = 0; s < S; ++s) {
-        for (int64_t qi = 0; qi < g1[s].n; ++qi) {
-            int64_t a = g1[s].ids[qi];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t a = 0; a < N; ++a) {
+        {
+            const int s = site1[a] - 1;
+            if (s < 0 || s >= S) continue;
             int c = chunk_of[a];
             const float* q = F1 + a * D;
             float ra = rho[a];
@@ -302,6 +308,38 @@ int orc_knn_ab(int D, int64_t nB, const float* B, int64_t nA, const float* A, in
         int64_t kk = nB < k ? nB : k;
         for (int j = 0; j < k; ++j) I_out[i * k + j] = (int32_t)(buf[j < kk ? j : kk - 1].col + 1);
         K_out[i] = (float)sqrt(buf[kk - 1].t);
+    }
+    free(buf);
+    return 0;
+}
+
+/* cuda_knn_full (cuda_knn_full.cu:27-100) and cuda_knn_ib (cuda_knn_ib.cu:27-97), exact:
+ * the candidates are the columns IB[0..nIB) of B (1-based, cuda_knn_ib.cu:59; IB NULL: every
+ * column of B), ordered by (d2, B column); fewer than k are padded with the last one.
+ * K_all (k x nA, may be NULL) = float32(sqrt(d2)) of every listed neighbour (cuda_knn_full.cu:97-100),
+ * K_out (nA, may be NULL) = that of the k-th (cuda_knn_ib.cu:90), I_out (k x nA) = 1-based B columns. */
+int orc_knn_ab_list(int D, int64_t nB, const float* B, int64_t nIB, const int32_t* IB, int64_t nA,
+                    const float* A, int k, float* K_all, float* K_out, int32_t* I_out) {
+    const int64_t n = IB ? nIB : nB;
+    if (n <= 0 || k <= 0) return 1;
+    for (int64_t j = 0; IB && j < n; ++j)
+        if (IB[j] < 1 || IB[j] > nB) return 2;
+    orc_cand* buf = (orc_cand*)malloc(sizeof(orc_cand) * (size_t)n);
+    for (int64_t i = 0; i < nA; ++i) {
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t col = IB ? (int64_t)IB[j] - 1 : j;
+            buf[j].t = orc_d2(A + i * D, B + col * D, D);
+            buf[j].key = col;
+            buf[j].col = col;
+        }
+        qsort(buf, (size_t)n, sizeof(orc_cand), cmp_cand);
+        const int64_t kk = n < k ? n : k;
+        for (int j = 0; j < k; ++j) {
+            const orc_cand* c = &buf[j < kk ? j : kk - 1];
+            I_out[i * k + j] = (int32_t)(c->col + 1);
+            if (K_all) K_all[i * k + j] = (float)sqrt(c->t);
+        }
+        if (K_out) K_out[i] = (float)sqrt(buf[kk - 1].t);
     }
     free(buf);
     return 0;

@@ -49,6 +49,8 @@ def oracle():
     lib.orc_delta_site.argtypes = [ctypes.c_int, ctypes.c_int64, _f32, _i32, _i64, ctypes.c_int, _u8,
                                    ctypes.c_int64, ctypes.c_int64, _f32, _f32, _i32]
     lib.orc_knn_ab.argtypes = [ctypes.c_int, ctypes.c_int64, _f32, ctypes.c_int64, _f32, ctypes.c_int, _f32, _i32]
+    lib.orc_knn_ab_list.argtypes = [ctypes.c_int, ctypes.c_int64, _f32, ctypes.c_int64, _i32, ctypes.c_int64,
+                                    _f32, ctypes.c_int, _f32, _f32, _i32]
     lib.orc_delta_knn_ab.argtypes = [ctypes.c_int, ctypes.c_int64, _f32, _f32, ctypes.c_int64, _f32, _f32,
                                      _f32, _i32]
     lib.orc_rho_drift_dc.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _i32, _f32, _f32,
@@ -126,6 +128,20 @@ def knn_ab(B, A, k):
     assert lib.orc_knn_ab(B.shape[1], B.shape[0], _p(B, _f32), A.shape[0], _p(A, _f32), k, _p(K, _f32),
                           _p(I, _i32)) == 0
     return K, I
+
+
+def knn_ab_list(B, A, k, IB=None):
+    """cuda_knn_full / cuda_knn_ib, exact: (K_all (nA, k), K (nA,), I (nA, k)); IB 1-based columns of B."""
+    lib = oracle()
+    B = np.ascontiguousarray(B, np.float32)
+    A = np.ascontiguousarray(A, np.float32)
+    ib = None if IB is None else np.ascontiguousarray(IB, np.int32)
+    Ka = np.zeros((A.shape[0], k), np.float32)
+    K = np.zeros(A.shape[0], np.float32)
+    I = np.zeros((A.shape[0], k), np.int32)
+    assert lib.orc_knn_ab_list(B.shape[1], B.shape[0], _p(B, _f32), 0 if ib is None else len(ib), _p(ib, _i32),
+                               A.shape[0], _p(A, _f32), k, _p(Ka, _f32), _p(K, _f32), _p(I, _i32)) == 0
+    return Ka, K, I
 
 
 def delta_knn_ab(B, RB, A, RA):

@@ -100,12 +100,72 @@ def test_layout_from_files_equals_in_memory_layout(tmp_path):
     np.testing.assert_array_equal(got["site_n_primary"], lay["site_n_primary"])
 
 
+def _read_knn_as_reference(path, knn):
+    """load_bin_(vcFile_knn_site1, 'int64', [knn, n]) (matlab/irc2.m:3395): raw little-endian int64,
+    column-major knn x n, read here without the module under test."""
+    raw = open(path, "rb").read()
+    assert len(raw) % (8 * knn) == 0
+    n = len(raw) // (8 * knn)
+    return np.frombuffer(raw, dtype="<i8").reshape((n, knn)).T
+
+
 def test_knn_files_append_per_page(tmp_path):
     rng = np.random.default_rng(7)
     prefix = str(tmp_path / "rec")
-    p1 = rng.integers(1, 1000, size=(5, 30)).astype(np.int32)
-    p2 = rng.integers(1, 1000, size=(5, 12)).astype(np.int32)
-    IF.save_miKnn_site_(prefix, 3, p1)
+    # ids above 2^31 exist in int64 files (the reference type, matlab/irc2.m:3670)
+    p1 = rng.integers(1, 1 << 40, size=(5, 30)).astype(np.int64)
+    p2 = rng.integers(1, 1000, size=(5, 12)).astype(np.int32)      # an int32 page is widened on write
+    path = IF.save_miKnn_site_(prefix, 3, p1)
     IF.save_miKnn_site_(prefix, 3, p2, fAppend=True)
-    back = IF.load_miKnn_site_(prefix, 3, 5)
-    np.testing.assert_array_equal(back, np.concatenate([p1, p2], axis=1))
+    want = np.concatenate([p1, p2.astype(np.int64)], axis=1)
+    assert (tmp_path / "rec_knn_3.irc").stat().st_size == 8 * want.size
+    np.testing.assert_array_equal(_read_knn_as_reference(path, 5), want)
+    np.testing.assert_array_equal(IF.load_miKnn_site_(prefix, 3, 5), want)
+
+
+def test_sort_cache_round_trip(tmp_path):
+    """save_sort_ writes rho, delta, nneigh, ordrho back to back in struct_save_bin_'s layout
+    (matlab/irc2.m:1733-1753) and S_var beside them; the blob is read here from S_var's classes and
+    sizes alone (as struct_load_bin_ does, matlab/irc2.m:1775-1805), and by load_sort_."""
+    from scipy.io import loadmat
+    from oracle import oracle as O
+    ds = synth.make_dataset(2500, 3, 6, 4, 3, k=7, drift_step=0.3, n_links=2, band=1, seed=8)
+    r = O.sort_batch(ds)
+    rho = (r["rho"] / r["rho"].max()).astype(np.float32)
+    S_clu = dict(rho=rho, delta=r["delta"], nneigh=r["nneigh"], ordrho=irc2.ordrho_(rho), nFeatures=6)
+    f = str(tmp_path / "sort_irc.mat")
+    blob, S_var = IF.save_sort_(S_clu, f)
+    assert blob.endswith("sort.irc")
+    m = loadmat(f, squeeze_me=True, simplify_cells=True)
+    sv = m["S_var"]
+    assert list(sv["csName_var"]) == ["rho", "delta", "nneigh", "ordrho"]
+    assert list(sv["csType_var"]) == ["single", "single", "int64", "double"]
+    raw = open(blob, "rb").read()
+    pos = 0
+    width = {"single": 4, "int64": 8, "double": 8}
+    code = {"single": "<f4", "int64": "<i8", "double": "<f8"}
+    for name, typ, dimm in zip(sv["csName_var"], sv["csType_var"], sv["cDimm_var"]):
+        n = int(np.prod(dimm))
+        assert list(np.ravel(dimm)) == [2500, 1]
+        got = np.frombuffer(raw[pos:pos + n * width[typ]], dtype=code[typ])
+        pos += n * width[typ]
+        np.testing.assert_array_equal(got, np.ravel(S_clu[name]), err_msg=name)
+    assert pos == len(raw)
+    back = IF.load_sort_(f)
+    for name in ("rho", "delta", "nneigh", "ordrho"):
+        np.testing.assert_array_equal(np.ravel(back[name]), np.ravel(S_clu[name]))
+        assert back[name].dtype == np.asarray(S_clu[name]).dtype
+    assert int(back["nFeatures"]) == 6
+
+
+def test_ordrho_matches_matlab_descending_sort():
+    # MATLAB's sort(v, 'descend') is stable and puts NaN first; rho / max(rho) is NaN where rho and
+    # max(rho) are both Inf (duplicate points, d_k = 0)
+    v = np.array([0.5, np.nan, 1.0, 0.5, 0.0, np.nan, 1.0, -0.0, 0.25], np.float32)
+    want = [2, 6, 3, 7, 1, 4, 9, 5, 8]
+    assert list(irc2.ordrho_(v).astype(int)) == want
+    rng = np.random.default_rng(2)
+    v = rng.integers(0, 5, size=300).astype(np.float32)
+    v[rng.random(300) < 0.1] = np.nan
+    key = lambda i: (0 if np.isnan(v[i]) else 1, 0.0 if np.isnan(v[i]) else -float(v[i]), i)
+    assert list(irc2.ordrho_(v).astype(int) - 1) == sorted(range(300), key=key)

@@ -1,0 +1,133 @@
+"""Full-size parity on the GPU for the benchmark workloads: the whole recording runs in one batch
+call, the oracle recomputes whole sites (a site's results depend only on its own columns for
+primary-only data, so a sub-recording of those sites reproduces them exactly), and size-independent
+properties are checked over every spike: self first in every kNN list, kNN lists sorted by distance,
+the denser neighbour really denser, rho = 1/d_k.
+
+  drift-10M      384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30 (16 sites, both forms)
+  lognormal-10M  lognormal site sizes (500..600k spikes) and Dirichlet chunk shares (3 sites, one
+                 from the top decile of site sizes; both forms)
+  multiday-100M  100M spikes, 2000 chunks (3 sites)
+"""
+import numpy as np
+import pytest
+import torch
+
+from ironclust_amd import irc2, libirc, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FORMS = {"bf16": libirc.PREC_BF16_SCREEN, "fp32": libirc.PREC_FP32}
+
+
+def _run(ds, prec, k):
+    lay = irc2.site_layout(ds)
+    N = ds["n_spikes"]
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"], n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"),
+               dk=torch.empty(N, dtype=torch.float32, device="cuda"))
+    ctx = irc2.get_ctx(0)
+    ctx.rho_delta_batch_device(dev, out, libirc.make_options(prec))
+    torch.cuda.synchronize()
+    del lay, dev
+    return out, ctx.stats()
+
+
+def _subset(ds, sites):
+    """The spikes whose primary site is in `sites`, renumbered in time order (gathered on the GPU, so
+    the full feature matrix never comes to the host); returns (sub-recording, original 1-based ids)."""
+    s1 = ds["site1"]
+    keep = torch.isin(s1, torch.as_tensor(np.asarray(sites), dtype=s1.dtype, device=s1.device))
+    orig_t = torch.nonzero(keep).flatten()
+    orig = orig_t.cpu().numpy() + 1
+    lim = np.asarray(ds["lim"])
+    chunk = np.searchsorted(lim, orig, side="right") - 1
+    C = len(lim) - 1
+    sub_lim = np.searchsorted(chunk, np.arange(C + 1), side="left") + 1
+    sub = dict(n_spikes=len(orig), n_sites=ds["n_sites"], D=ds["D"], n_drift=C, k=ds["k"],
+               F1=ds["F1"][orig_t].cpu().numpy(), F2=None, site1=s1[orig_t].cpu().numpy().astype(np.int32),
+               site2=None, lim=sub_lim.astype(np.int64), link=np.asarray(ds["link"]))
+    return sub, orig
+
+
+def _check_sites(ds, out, sites):
+    sub, orig = _subset(ds, sites)
+    r = O.sort_batch(sub, nthreads=16)
+    rows = torch.from_numpy(orig - 1).cuda()
+    np.testing.assert_array_equal(out["knn"][rows].cpu().numpy(), orig[r["knn"] - 1])
+    np.testing.assert_array_equal(out["rho"][rows].cpu().numpy(), r["rho"])
+    np.testing.assert_array_equal(out["nneigh"][rows].cpu().numpy(), orig[r["nneigh"] - 1])
+    np.testing.assert_array_equal(out["delta"][rows].cpu().numpy(), r["delta"])
+    return len(orig)
+
+
+def _check_invariants(ds, out):
+    N, k = ds["n_spikes"], ds["k"]
+    ids = torch.arange(1, N + 1, device="cuda")
+    knn, rho = out["knn"], out["rho"]
+    assert bool((knn[:, 0] == ids).all())
+    assert bool(((knn >= 1) & (knn <= N)).all())
+    has = out["nneigh"] != ids
+    assert bool((rho[out["nneigh"][has] - 1] > rho[has]).all())
+    assert bool(torch.isfinite(out["delta"]).all())
+    # rho = float32(1/d_k) with d_k = float32(sqrt(d2_k)) reported beside it (within one rounding)
+    rel = (rho.double() * out["dk"].double() - 1.0).abs()
+    assert float(rel.max()) < 1e-6
+    # kNN lists ascending in distance, checked in FP64 on a strided sample of spikes
+    F1 = ds["F1"]
+    samp = torch.arange(0, N, max(1, N // 20000), device="cuda")
+    q = F1[samp].double()
+    prev = torch.zeros(len(samp), dtype=torch.float64, device="cuda")
+    for j in range(k):
+        d = ((F1[knn[samp, j] - 1].double() - q) ** 2).sum(1)
+        assert bool((d >= prev).all()), j
+        prev = d
+
+
+def _site_sizes(ds):
+    return torch.bincount(ds["site1"].long(), minlength=ds["n_sites"] + 1)[1:].cpu().numpy()
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp32"])
+def test_drift_10m_sixteen_sites(form):
+    ds = synth.make_workload("drift-10M", device="cuda")
+    out, st = _run(ds, FORMS[form], ds["k"])
+    assert st["precision_used"] == FORMS[form]
+    assert st["n_screen_fallback"] < 1e-4 * ds["n_spikes"], st
+    size = _site_sizes(ds)
+    order = np.argsort(size, kind="stable") + 1
+    # the densest and sparsest sites, and fourteen spread over the rest
+    sites = sorted(set([int(order[-1]), int(order[0])] + [int(x) for x in order[np.linspace(1, 382, 14).astype(int)]]))
+    assert len(sites) == 16
+    _check_sites(ds, out, sites)
+    _check_invariants(ds, out)
+
+
+@pytest.mark.parametrize("form", ["bf16", "fp32"])
+def test_lognormal_10m_sites_and_invariants(form):
+    ds = synth.make_workload("lognormal-10M", device="cuda")
+    out, st = _run(ds, FORMS[form], ds["k"])
+    assert st["precision_used"] == FORMS[form]
+    size = _site_sizes(ds)
+    order = np.argsort(size, kind="stable")
+    top = order[int(0.9 * len(order))]                 # the smallest site of the top decile
+    sites = [int(order[0]) + 1, int(order[len(order) // 2]) + 1, int(top) + 1]
+    n = _check_sites(ds, out, sites)
+    assert n > size[top]
+    _check_invariants(ds, out)
+
+
+def test_multiday_100m_sites_and_invariants():
+    ds = synth.make_workload("multiday-100M", device="cuda")
+    out, st = _run(ds, libirc.PREC_BF16_SCREEN, ds["k"])
+    assert st["n_queries"] == 100_000_000
+    assert st["n_screen_fallback"] < 1e-4 * ds["n_spikes"], st
+    size = _site_sizes(ds)
+    order = np.argsort(size, kind="stable")
+    _check_sites(ds, out, [int(order[0]) + 1, int(order[-1]) + 1])
+    _check_invariants(ds, out)

@@ -169,39 +169,6 @@ def test_isolation_and_merge_match_oracle():
         np.testing.assert_array_equal(irc2.knn_overlap_merge_(clu, knn, th), A.knn_overlap_merge(clu, knn, th))
 
 
-def test_full_size_site_subset_bf16():
-    """drift-10M (384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30) on the GPU; the
-    oracle recomputes three whole sites (their results do not depend on other sites)."""
-    import torch
-    from tests.helpers import subset_sites
-    ds = synth.make_workload("drift-10M", device="cuda")
-    lay = irc2.site_layout(ds)
-    N, k = ds["n_spikes"], ds["k"]
-    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
-               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"], n_spikes=N, k=k)
-    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
-               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
-               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
-               knn=torch.empty((N, k), dtype=torch.int32, device="cuda"))
-    ctx = irc2.get_ctx(0)
-    ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
-    torch.cuda.synchronize()
-    host = dict(ds, F1=ds["F1"].cpu(), site1=ds["site1"].cpu())
-    sub, orig = subset_sites(host, [1, 200, 384])
-    del host
-    r = O.sort_batch(sub)
-    rows = torch.from_numpy(orig - 1).cuda()
-    np.testing.assert_array_equal(out["knn"][rows].cpu().numpy(), orig[r["knn"] - 1])
-    np.testing.assert_array_equal(out["rho"][rows].cpu().numpy(), r["rho"])
-    np.testing.assert_array_equal(out["nneigh"][rows].cpu().numpy(), orig[r["nneigh"] - 1])
-    np.testing.assert_array_equal(out["delta"][rows].cpu().numpy(), r["delta"])
-    # whole-recording properties: self first, denser neighbour, sorted distances
-    ids = torch.arange(1, N + 1, device="cuda")
-    assert bool((out["knn"][:, 0] == ids).all())
-    has = out["nneigh"] != ids
-    assert bool((out["rho"][out["nneigh"][has] - 1] > out["rho"][has]).all())
-
-
 def test_graph_replay_matches_eager():
     """IRC_FLAG_GRAPH: the first call runs and captures the device phase, later calls replay it.
     Every replay (with unrelated eager work and synchronizations in between) must equal the eager
@@ -216,7 +183,7 @@ def test_graph_replay_matches_eager():
     out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
                delta=torch.empty(N, dtype=torch.float32, device="cuda"),
                nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
-               knn=torch.empty((N, k), dtype=torch.int32, device="cuda"))
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
     ctx = libirc.Context(0)
     try:
         for feats_round in range(2):
@@ -405,3 +372,178 @@ def test_mfma_lane_layout(ctx):
     second = np.where(l < 32, np.roll(acc0, -32, axis=1), acc1)
     np.testing.assert_array_equal(out[2], first, err_msg="permlane32_swap first result")
     np.testing.assert_array_equal(out[3], second, err_msg="permlane32_swap second result")
+
+
+def _dev_batch(ds, k):
+    import torch
+    lay = irc2.site_layout(ds)
+    N = ds["n_spikes"]
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"], n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+    return dev, out
+
+
+def test_graph_replay_after_buffer_growth():
+    """A replay must not run on context buffers that another call reallocated: capture with
+    IRC_FLAG_GRAPH, grow buffers with other calls (a larger A/B kNN, an FP32 batch on a larger
+    recording), replay, and compare with the eager call and the oracle."""
+    import torch
+    ds = synth.make_dataset(20000, 4, 12, 8, 5, k=10, drift_step=0.3, n_links=3, band=2, seed=71, device="cuda")
+    big = synth.make_dataset(60000, 4, 12, 8, 5, k=16, drift_step=0.3, n_links=3, band=2, seed=72, device="cuda")
+    dev, out = _dev_batch(ds, 10)
+    bdev, bout = _dev_batch(big, 16)
+    ctx = libirc.Context(0)
+    try:
+        g = libirc.make_options(libirc.PREC_BF16_SCREEN, flags=libirc.FLAG_GRAPH)
+        ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
+        ref = {kk: v.clone() for kk, v in out.items()}
+        ctx.rho_delta_batch_device(dev, out, g)            # runs and captures
+        ctx.rho_delta_batch_device(dev, out, g)            # replays
+        assert ctx.stats()["ms_graph"] > 0.0
+        rng = np.random.default_rng(3)
+        B = rng.standard_normal((30000, 12)).astype(np.float32)
+        A = rng.standard_normal((40000, 12)).astype(np.float32)
+        for grow in range(2):
+            if grow == 0:
+                ctx.knn_ab(B, A, 30)                               # grows the kNN and plan buffers
+            else:
+                ctx.rho_delta_batch_device(bdev, bout, libirc.make_options(libirc.PREC_FP32))
+            for v in out.values():
+                v.zero_()
+            torch.cuda.synchronize()
+            ctx.rho_delta_batch_device(dev, out, g)
+            for kk in ref:
+                assert torch.equal(out[kk], ref[kk]), (grow, kk)
+            ctx.rho_delta_batch_device(dev, out, g)            # and the fresh capture replays
+            assert ctx.stats()["ms_graph"] > 0.0
+            for kk in ref:
+                assert torch.equal(out[kk], ref[kk]), (grow, kk)
+        r = O.sort_batch(dict(ds, F1=ds["F1"].cpu(), site1=ds["site1"].cpu()))
+        np.testing.assert_array_equal(out["knn"].cpu().numpy(), r["knn"])
+        np.testing.assert_array_equal(out["delta"].cpu().numpy(), r["delta"])
+    finally:
+        ctx.close()
+
+
+def test_site_launches_match_persistent_launch():
+    ds = synth.make_dataset(30000, 6, 14, 10, 5, k=12, drift_step=0.3, n_links=4, band=3, seed=73, lognormal=True,
+                            device="cuda")
+    dev, out = _dev_batch(ds, 12)
+    ctx = irc2.get_ctx(0)
+    ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
+    ref = {kk: v.clone() for kk, v in out.items()}
+    ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN,
+                                                             flags=libirc.FLAG_SITE_LAUNCHES))
+    for kk in ref:
+        assert bool((out[kk] == ref[kk]).all()), kk
+    with pytest.raises(libirc.IrcError):
+        ctx.rho_delta_batch_device(dev, out, libirc.make_options(
+            libirc.PREC_BF16_SCREEN, flags=libirc.FLAG_SITE_LAUNCHES | libirc.FLAG_GRAPH))
+
+
+@pytest.mark.parametrize("prec", [libirc.PREC_BF16_SCREEN, libirc.PREC_FP32])
+def test_autotune_choice_reproduces_default(prec, tmp_path):
+    """irc_autotune: every trial setting gives the base setting's outputs bit for bit, the choice is
+    one of them, and batch calls with the choice, with IRC_FLAG_AUTOTUNE and captured in a graph after
+    tuning all equal the oracle."""
+    import torch
+    from ironclust_amd import autotune
+    ds = synth.make_dataset(40000, 5, 16, 10, 6, k=20, drift_step=0.3, n_links=4, band=3, seed=74, device="cuda")
+    dev, out = _dev_batch(ds, 20)
+    ctx = libirc.Context(0)
+    try:
+        choice, rep = autotune.tune(ctx, dev, out, precision=prec, cache=str(tmp_path / "tune.json"), force=True)
+        assert len(rep["trials"]) >= 5
+        assert all(t["identical"] for t in rep["trials"]), rep["trials"]
+        r = O.sort_batch(dict(ds, F1=ds["F1"].cpu(), site1=ds["site1"].cpu()))
+        for opts in (libirc.make_options(prec, **choice), libirc.make_options(prec, flags=libirc.FLAG_AUTOTUNE),
+                     libirc.make_options(prec, flags=libirc.FLAG_GRAPH, **choice),
+                     libirc.make_options(prec, flags=libirc.FLAG_GRAPH, **choice)):
+            for v in out.values():
+                v.zero_()
+            ctx.rho_delta_batch_device(dev, out, opts)
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(out["knn"].cpu().numpy(), r["knn"])
+            np.testing.assert_array_equal(out["rho"].cpu().numpy(), r["rho"])
+            np.testing.assert_array_equal(out["nneigh"].cpu().numpy(), r["nneigh"])
+            np.testing.assert_array_equal(out["delta"].cpu().numpy(), r["delta"])
+        again, rep2 = autotune.tune(ctx, dev, out, precision=prec, cache=str(tmp_path / "tune.json"))
+        assert rep2["cached"] and again == choice
+    finally:
+        ctx.close()
+
+
+def test_ab_full_and_indirect_forms(ctx):
+    rng = np.random.default_rng(8)
+    B = rng.standard_normal((700, 9)).astype(np.float32)
+    A = rng.standard_normal((300, 9)).astype(np.float32)
+    for k in (1, 6, 30):
+        K_g, I_g = ctx.knn_ab_full(B, A, k)
+        Ka, _, I_r = O.knn_ab_list(B, A, k)
+        np.testing.assert_array_equal(I_g, I_r)
+        np.testing.assert_array_equal(K_g, Ka)
+    IB = np.concatenate([rng.permutation(700)[:250] + 1, [17, 17, 3]]).astype(np.int32)
+    for k in (1, 9, 30):
+        K_g, I_g = ctx.knn_ab_ib(B, IB, A, k)
+        _, K_r, I_r = O.knn_ab_list(B, A, k, IB=IB)
+        np.testing.assert_array_equal(I_g, I_r)
+        np.testing.assert_array_equal(K_g, K_r)
+    # fewer listed candidates than k: padded with the last one
+    K_g, I_g = ctx.knn_ab_ib(B, np.array([5, 2], np.int32), A, 4)
+    _, K_r, I_r = O.knn_ab_list(B, A, 4, IB=np.array([5, 2], np.int32))
+    np.testing.assert_array_equal(I_g, I_r)
+    np.testing.assert_array_equal(K_g, K_r)
+    with pytest.raises(libirc.IrcError):
+        ctx.knn_ab_ib(B, np.array([0, 701], np.int32), A, 4)
+    mK, mI = irc2.cuda_knn_full_(B.T, A.T, {"knn": 6})
+    assert mK.shape == (6, 300) and mI.shape == (6, 300)
+    np.testing.assert_array_equal(mI.T, O.knn_ab_list(B, A, 6)[2])
+    vK, vI = irc2.cuda_knn_ib_(B.T, IB, A.T, {"knn": 5})
+    np.testing.assert_array_equal(vI.T, O.knn_ab_list(B, A, 5, IB=IB)[2])
+
+
+def test_sort_chain_matches_oracle(tmp_path):
+    """sort_: drift linkage on the GPU, the batch, rho / max(rho) and ordrho (the NaN rule of ordrho_
+    is pinned on the CPU), with the features in memory and from detect files (two loads, secondary
+    sites; the per-site _knn_ files are written on the way); the cache round-trips."""
+    import torch
+    from ironclust_amd import irc_files as IF
+    from tests.helpers import write_detect_files
+    rng = np.random.default_rng(5)
+    N = 6000
+    ds = synth.make_dataset(N, 4, 6, 1, 5, k=8, seed=75, f2=True)
+    amp = rng.gamma(2.0, 20.0, size=N).astype(np.float32)
+    pos = np.stack([np.zeros(N), rng.uniform(0, 400, size=N)], 1).astype(np.float32)
+    P = {"nTime_batch": 3, "nTime_drift": 12, "nQuantile_drift": 6, "nSites": 8, "knn": 8, "precision": "bf16"}
+    S0 = dict(vrAmp_spk=amp, mrPos_spk=pos, F1=ds["F1"].numpy(), viSite_spk=ds["site1"].numpy(),
+              F2=ds["F2"].numpy(), viSite2_spk=ds["site2"].numpy(), nSites=4)
+    S_clu = irc2.sort_(S0, P)
+    S_drift = S_clu["S_drift"]
+    assert S_drift["mlDrift"].shape == (12, 12)
+    r = O.sort_batch(dict(ds, lim=S_drift["viLim_drift"], link=S_drift["mlDrift"].astype(np.uint8), n_drift=12))
+    rho_n = (r["rho"] / r["rho"].max()).astype(np.float32)
+    np.testing.assert_array_equal(S_clu["rho"], rho_n)
+    np.testing.assert_array_equal(S_clu["delta"], r["delta"])
+    np.testing.assert_array_equal(S_clu["nneigh"], r["nneigh"])
+    assert S_clu["ordrho"].dtype == np.float64
+    want = sorted(range(N), key=lambda i: (-float(rho_n[i]), i))
+    np.testing.assert_array_equal(S_clu["ordrho"].astype(np.int64) - 1, want)
+    prefix = str(tmp_path / "rec")
+    cc1, cc2, dimm, ml = write_detect_files(ds, prefix, 2, 3)
+    S0f = dict(vrAmp_spk=amp, mrPos_spk=pos, vcFile_prm=prefix, ccviSpk_site_load=cc1, ccviSpk_site2_load=cc2,
+               dimm_fet=dimm, mlPc=ml)
+    S_f = irc2.sort_(S0f, dict(P, vcFile_prm=prefix))
+    for kk in ("rho", "delta", "nneigh", "ordrho"):
+        np.testing.assert_array_equal(S_f[kk], S_clu[kk])
+    s1 = ds["site1"].numpy()
+    for s in range(1, 5):
+        np.testing.assert_array_equal(IF.load_miKnn_site_(prefix, s, 8), r["knn"][np.flatnonzero(s1 == s)].T)
+    f = str(tmp_path / "sort_irc.mat")
+    IF.save_sort_(S_f, f)
+    back = IF.load_sort_(f)
+    for kk in ("rho", "delta", "nneigh", "ordrho"):
+        np.testing.assert_array_equal(np.ravel(back[kk]), S_f[kk])

@@ -180,6 +180,33 @@ def test_invariants_on_drift_data():
     assert r["nneigh"][top] == top + 1
 
 
+def test_ab_list_forms_brute_force():
+    # cuda_knn_full (every neighbour's distance) and cuda_knn_ib (candidates listed by IB, 1-based,
+    # with a repeated column and columns out of order; indices reported as columns of B)
+    rng = np.random.default_rng(4)
+    B = rng.integers(-3, 4, size=(70, 5)).astype(np.float32)
+    A = rng.integers(-3, 4, size=(30, 5)).astype(np.float32)
+    d = ((A[:, None, :].astype(np.float64) - B[None].astype(np.float64)) ** 2).sum(2)
+    Ka, K, I = O.knn_ab_list(B, A, 7)
+    for i in range(30):
+        o = np.lexsort((np.arange(70), d[i]))[:7]
+        assert list(I[i] - 1) == list(o)
+        assert list(Ka[i]) == [np.float32(np.sqrt(d[i, j])) for j in o]
+        assert K[i] == Ka[i, -1]
+    IB = np.concatenate([rng.permutation(70)[:25] + 1, [5, 5]]).astype(np.int32)
+    Ka, K, I = O.knn_ab_list(B, A, 9, IB=IB)
+    for i in range(30):
+        cols = IB - 1
+        o = np.lexsort((cols, d[i, cols]))[:9]
+        assert list(I[i] - 1) == list(cols[o])
+        assert K[i] == np.float32(np.sqrt(d[i, cols[o[-1]]]))
+    # fewer listed candidates than k: padded with the last one
+    Ka, K, I = O.knn_ab_list(B, A, 5, IB=np.array([3, 9], np.int32))
+    for i in range(30):
+        o = np.lexsort((np.array([2, 8]), d[i, [2, 8]]))
+        assert list(I[i] - 1) == [2 if o[0] == 0 else 8] + [2 if o[1] == 0 else 8] * 4
+
+
 def test_ab_and_dc_forms_brute_force():
     rng = np.random.default_rng(3)
     B = rng.integers(-3, 4, size=(60, 4)).astype(np.float32)
