@@ -859,14 +859,12 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     // The replay needs no set-up when the captured graph still matches: same descriptor, no context
     // buffer reallocated since the capture (buf_gen), and the shared planning buffers still hold this
     // descriptor's runs and queries (no other call has written them since, desc_key).
-    const bool fast = graph && ctx->graph_exec && ctx->graph_key == h && ctx->graph_gen == ctx->buf_gen &&
-                      ctx->desc_key == h;
-    ctx->desc_key = 0;
-
     const float* dF = b->F;
     const int* dspk = b->col_spike;
     if (!dev) {
+        // both copies allocated before their addresses are taken
         CHK(ensure(ctx, ctx->b_F, sizeof(float) * (size_t)n_cols * D));
+        CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
         dF = ptr<float>(ctx->b_F);
         dspk = ptr<int>(ctx->b_colspk);
     }
@@ -895,6 +893,10 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
             o_dk = ptr<float>(ctx->b_out_dk);
         }
     }
+    // decided after this call's own allocations above, so a reallocation in them is seen too
+    const bool fast = graph && ctx->graph_exec && ctx->graph_key == h && ctx->graph_gen == ctx->buf_gen &&
+                      ctx->desc_key == h;
+    ctx->desc_key = 0;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = N;
     bool replayed = false;
@@ -925,7 +927,6 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
         CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
         CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
-        if (!dev) CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
         int64_t* d_site_ptr = ptr<int64_t>(ctx->b_small);
         int64_t* d_site_np = d_site_ptr + (S + 1);
         int64_t* d_prim = d_site_np + S;

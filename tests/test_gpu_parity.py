@@ -387,6 +387,24 @@ def _dev_batch(ds, k):
     return dev, out
 
 
+def test_host_buffers_after_device_calls():
+    """A fresh context used in device-pointer form first, then with host buffers: the host form's own
+    copies of F and col_spike must be allocated before the set-up kernels read them."""
+    ds = synth.make_dataset(5000, 3, 8, 4, 3, k=7, drift_step=0.3, n_links=2, band=1, seed=70, device="cuda")
+    dev, out = _dev_batch(ds, 7)
+    ctx = libirc.Context(0)
+    try:
+        ctx.rho_delta_batch_device(dev, out, libirc.make_options(libirc.PREC_BF16_SCREEN))
+        host = {kk: (v.cpu().numpy() if hasattr(v, "cpu") else v) for kk, v in irc2.site_layout(ds).items()}
+        r = ctx.rho_delta_batch(host, 7, opts=libirc.make_options(libirc.PREC_BF16_SCREEN))
+        ref = O.sort_batch(dict(ds, F1=ds["F1"].cpu(), site1=ds["site1"].cpu()))
+        np.testing.assert_array_equal(r["knn"], ref["knn"])
+        np.testing.assert_array_equal(r["delta"], ref["delta"])
+        np.testing.assert_array_equal(out["knn"].cpu().numpy(), ref["knn"])
+    finally:
+        ctx.close()
+
+
 def test_graph_replay_after_buffer_growth():
     """A replay must not run on context buffers that another call reallocated: capture with
     IRC_FLAG_GRAPH, grow buffers with other calls (a larger A/B kNN, an FP32 batch on a larger
