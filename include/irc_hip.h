@@ -189,6 +189,20 @@ typedef struct {
 irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* batch, const irc_options* opt,
                                irc_outputs* out);
 
+/* irc_rho_delta_batch_noalloc: irc_rho_delta_batch with no device memory allocation (no counterpart in
+ * the reference).  For callers that cannot call hipMalloc/hipFree: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  Every device buffer the call needs (plan, packed
+ * operands, candidate lists, kNN lists, and in host mode the staging copies) is carved from a 64 MiB
+ * array held by each GPU the library runs on, for the life of the process; a problem that needs more
+ * (about 55 000 spikes at D = 42, k = 30 with 384 sites x 200 chunks) fails with IRC_E_UNSUPPORTED before
+ * any output is written.  Inputs, outputs and options are those of irc_rho_delta_batch (host or device
+ * memory); IRC_FLAG_GRAPH, IRC_FLAG_AUTOTUNE and IRC_FLAG_SITE_LAUNCHES are rejected.  All calls on one
+ * GPU share its array, which keeps the last call's contents: calls that overlap on one GPU (from
+ * several contexts or threads) race.  Host-side descriptor vectors (O(sites + links)) are the only
+ * allocations it makes. */
+irc_status irc_rho_delta_batch_noalloc(irc_ctx* ctx, const irc_batch* batch, const irc_options* opt,
+                                       irc_outputs* out);
+
 /* Autotuner of irc_rho_delta_batch (no reference counterpart: the reference fixes its launch shapes,
  * matlab/irc2.m:3755-3757, 3775).  Times the call on the caller's batch and outputs with each
  * candidate setting of the over-fetch depth (k_over), the screen's persistent grid (waves_per_cu), the

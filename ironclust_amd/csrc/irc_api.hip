@@ -23,11 +23,43 @@
 #include "irc_screen.h"
 #include "../../include/irc_hip.h"
 
+// Scratch of irc_rho_delta_batch_noalloc: every buffer such a call needs is carved from this array.
+// Each GPU holds its own copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (irc_ctx::scratch), never through its name.
+#define IRC_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) char scratch[IRC_NOALLOC_SCRATCH_BYTES];
+
 namespace {
 
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+};
+
+struct Bufs {
+    DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
+    DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
+        b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
+    DevBuf b_scr[6];   // screen: [0] hi, [1] lo, [2] norms + run maxima, [3] items, [4] fallback/spill lists, [5] counters
+    DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
+    DevBuf b_plan;                                         // device-side work planning
+    DevBuf b_drift;                                        // drift linkage scratch
+    DevBuf b_assign;                                       // chain labelling scratch
+    DevBuf b_siteseg;     // per-site segment tables (IRC_FLAG_SITE_LAUNCHES)
+    DevBuf b_tune;        // the autotuner's copy of the base setting's outputs
+    DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
+                          // re-rank, [2] pairs, [3] queries the re-rank's second pass redid, [4] queries
+                          // whose denser neighbour was outside the kNN list
+    // every buffer, for release
+    std::vector<DevBuf*> all() {
+        std::vector<DevBuf*> v{&b_F, &b_colspk, &b_feat, &b_key, &b_ret, &b_runb, &b_runl, &b_lptr, &b_lidx,
+                               &b_qcol, &b_qblk, &b_qout, &b_small, &b_flags, &b_knncol, &b_dk2, &b_rhoslot,
+                               &b_rhocol, &b_unres, &b_out_delta, &b_out_nn, &b_out_knn, &b_out_dk, &b_out_nn32,
+                               &b_out_knn32, &b_ncand, &b_cand_n, &b_cand_col, &b_cand_d, &b_cand_eps, &b_plan,
+                               &b_drift, &b_assign, &b_siteseg, &b_tune, &b_dstat};
+        for (auto& b : b_scr) v.push_back(&b);
+        return v;
+    }
 };
 
 }  // namespace
@@ -42,20 +74,12 @@ struct irc_ctx {
     int flags = 0;       // irc_options.flags of the current call
     int waves_per_cu = 8;  // persistent grid of the screen
     int large_ncand = 8192;
-    // device buffers, grown on demand and kept across calls
-    DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
-    DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
-        b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
-    DevBuf b_scr[6];   // screen: [0] hi, [1] lo, [2] norms + run maxima, [3] items, [4] fallback/spill lists, [5] counters
-    DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
-    DevBuf b_plan;                                         // device-side work planning
-    DevBuf b_drift;                                        // drift linkage scratch
-    DevBuf b_assign;                                       // chain labelling scratch
-    DevBuf b_siteseg;     // per-site segment tables (IRC_FLAG_SITE_LAUNCHES)
-    DevBuf b_tune;        // the autotuner's copy of the base setting's outputs
-    DevBuf b_dstat;       // device counters of the call: [0] fallbacks after the screen, [1] after the
-                          // re-rank, [2] pairs, [3] queries the re-rank's second pass redid, [4] queries
-                          // whose denser neighbour was outside the kNN list
+    Bufs heap;           // device buffers, grown on demand and kept across calls
+    Bufs arena;          // the same buffers carved from the static scratch array (no-allocation calls)
+    Bufs* bb = &heap;    // the set in use
+    char* scratch = nullptr;   // device address of the scratch array on this context's device
+    size_t scratch_used = 0;   // bytes carved in the current no-allocation call
+    bool arena_on = false;
     bool dstat_screen = false;  // [0], [1] hold this call's screen counters
     bool capturing = false;     // the stream is being captured into a graph: no events, no host copies
     hipGraphExec_t graph_exec = nullptr;  // the batch call's device phase, for replays
@@ -95,6 +119,18 @@ irc_status fail(irc_ctx* c, irc_status st, const char* fmt, ...) {
 irc_status ensure(irc_ctx* ctx, DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (bytes <= b.cap) return IRC_OK;
+    if (ctx->arena_on) {
+        // a no-allocation call: carve from the scratch array (never freed; the set is reset per call)
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (ctx->scratch_used + need > IRC_NOALLOC_SCRATCH_BYTES)
+            return fail(ctx, IRC_E_UNSUPPORTED,
+                        "problem needs more than the %d MiB scratch of irc_rho_delta_batch_noalloc",
+                        (int)(IRC_NOALLOC_SCRATCH_BYTES >> 20));
+        b.p = ctx->scratch + ctx->scratch_used;
+        b.cap = need;
+        ctx->scratch_used += need;
+        return IRC_OK;
+    }
     if (b.p) {
         hipError_t e = hipFree(b.p);
         b.p = nullptr;
@@ -165,42 +201,42 @@ struct HostPlan {
 irc_status upload_plan(irc_ctx* ctx, const HostPlan& h, DevPlan& p) {
     const int64_t n = h.n_cols;
     ctx->desc_key = 0;   // the batch call's planning buffers are overwritten below
-    CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n * h.DP));
-    CHK(ensure(ctx, ctx->b_key, sizeof(int) * (size_t)n));
-    CHK(ensure(ctx, ctx->b_ret, sizeof(int) * (size_t)n));
-    CHK(ensure(ctx, ctx->b_runb, sizeof(int) * h.runb.size()));
-    CHK(ensure(ctx, ctx->b_runl, sizeof(int) * h.runl.size()));
-    CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * h.lptr.size()));
-    CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (h.lidx.size() + 1)));
-    CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * h.qcol.size()));
-    CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * h.qblk.size()));
-    CHK(ensure(ctx, ctx->b_qout, sizeof(int) * h.qout.size()));
+    CHK(ensure(ctx, ctx->bb->b_feat, sizeof(float) * (size_t)n * h.DP));
+    CHK(ensure(ctx, ctx->bb->b_key, sizeof(int) * (size_t)n));
+    CHK(ensure(ctx, ctx->bb->b_ret, sizeof(int) * (size_t)n));
+    CHK(ensure(ctx, ctx->bb->b_runb, sizeof(int) * h.runb.size()));
+    CHK(ensure(ctx, ctx->bb->b_runl, sizeof(int) * h.runl.size()));
+    CHK(ensure(ctx, ctx->bb->b_lptr, sizeof(int) * h.lptr.size()));
+    CHK(ensure(ctx, ctx->bb->b_lidx, sizeof(int) * (h.lidx.size() + 1)));
+    CHK(ensure(ctx, ctx->bb->b_qcol, sizeof(int) * h.qcol.size()));
+    CHK(ensure(ctx, ctx->bb->b_qblk, sizeof(int) * h.qblk.size()));
+    CHK(ensure(ctx, ctx->bb->b_qout, sizeof(int) * h.qout.size()));
     hipStream_t st = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->b_feat.p, h.feat.data(), sizeof(float) * h.feat.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_key.p, h.key.data(), sizeof(int) * h.key.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_ret.p, h.ret.data(), sizeof(int) * h.ret.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_runb.p, h.runb.data(), sizeof(int) * h.runb.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_runl.p, h.runl.data(), sizeof(int) * h.runl.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, h.lptr.data(), sizeof(int) * h.lptr.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_feat.p, h.feat.data(), sizeof(float) * h.feat.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_key.p, h.key.data(), sizeof(int) * h.key.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_ret.p, h.ret.data(), sizeof(int) * h.ret.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_runb.p, h.runb.data(), sizeof(int) * h.runb.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_runl.p, h.runl.data(), sizeof(int) * h.runl.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_lptr.p, h.lptr.data(), sizeof(int) * h.lptr.size(), hipMemcpyHostToDevice, st));
     if (!h.lidx.empty())
-        HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, h.lidx.data(), sizeof(int) * h.lidx.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_qcol.p, h.qcol.data(), sizeof(int) * h.qcol.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_qblk.p, h.qblk.data(), sizeof(int) * h.qblk.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_qout.p, h.qout.data(), sizeof(int) * h.qout.size(), hipMemcpyHostToDevice, st));
-    p.feat = ptr<float>(ctx->b_feat);
+        HIPCHK(hipMemcpyAsync(ctx->bb->b_lidx.p, h.lidx.data(), sizeof(int) * h.lidx.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_qcol.p, h.qcol.data(), sizeof(int) * h.qcol.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_qblk.p, h.qblk.data(), sizeof(int) * h.qblk.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_qout.p, h.qout.data(), sizeof(int) * h.qout.size(), hipMemcpyHostToDevice, st));
+    p.feat = ptr<float>(ctx->bb->b_feat);
     p.DP = h.DP;
     p.D = h.D;
     p.C = h.C;
     p.k = h.k;
-    p.col_key = ptr<int>(ctx->b_key);
-    p.col_ret = ptr<int>(ctx->b_ret);
-    p.run_beg = ptr<int>(ctx->b_runb);
-    p.run_len = ptr<int>(ctx->b_runl);
-    p.link_ptr = ptr<int>(ctx->b_lptr);
-    p.link_idx = ptr<int>(ctx->b_lidx);
-    p.q_col = ptr<int>(ctx->b_qcol);
-    p.q_blk = ptr<int>(ctx->b_qblk);
-    p.q_out = ptr<int>(ctx->b_qout);
+    p.col_key = ptr<int>(ctx->bb->b_key);
+    p.col_ret = ptr<int>(ctx->bb->b_ret);
+    p.run_beg = ptr<int>(ctx->bb->b_runb);
+    p.run_len = ptr<int>(ctx->bb->b_runl);
+    p.link_ptr = ptr<int>(ctx->bb->b_lptr);
+    p.link_idx = ptr<int>(ctx->bb->b_lidx);
+    p.q_col = ptr<int>(ctx->bb->b_qcol);
+    p.q_blk = ptr<int>(ctx->bb->b_qblk);
+    p.q_out = ptr<int>(ctx->bb->b_qout);
     p.nq = (int)h.qcol.size();
     p.n_blocks = (int)(h.runb.size() / 2);
     return IRC_OK;
@@ -372,8 +408,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     ctx->stats.n_rerank_overflow = 0;
     ctx->stats.n_rerank_second = 0;
     ctx->dstat_screen = false;
-    CHK(ensure(ctx, ctx->b_dstat, sizeof(int64_t) * 8));
-    HIPCHK(irc::launch_zero_i64(st, ptr<int64_t>(ctx->b_dstat), 8));
+    CHK(ensure(ctx, ctx->bb->b_dstat, sizeof(int64_t) * 8));
+    HIPCHK(irc::launch_zero_i64(st, ptr<int64_t>(ctx->bb->b_dstat), 8));
     // the screen's operand rows hold D + 2 values in at most 4 MFMA K-blocks of 16 (D <= 62);
     // wider features take the exact path
     const bool screen = (prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN) && irc::screen_kp(p.D) <= 64;
@@ -395,11 +431,11 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     kprime = kprime <= 32 ? 32 : (kprime <= 40 ? 40 : (kprime <= 48 ? 48 : 64));
     const int large_ncand = ctx->large_ncand;
     // [0, nq) exact-kernel list, [nq] its count, [nq + 1, 2 nq + 1) the first re-rank pass's overflows
-    CHK(ensure(ctx, ctx->b_scr[4], sizeof(int) * (2 * (size_t)nq + 1)));
-    CHK(ensure(ctx, ctx->b_scr[5], sizeof(int) * 32));
-    int* fb = ptr<int>(ctx->b_scr[4]);
+    CHK(ensure(ctx, ctx->bb->b_scr[4], sizeof(int) * (2 * (size_t)nq + 1)));
+    CHK(ensure(ctx, ctx->bb->b_scr[5], sizeof(int) * 32));
+    int* fb = ptr<int>(ctx->bb->b_scr[4]);
     int* nfb = fb + nq;
-    int* seg_ctr = ptr<int>(ctx->b_scr[5]);
+    int* seg_ctr = ptr<int>(ctx->bb->b_scr[5]);
     int* d_seg_beg = seg_ctr + 8;
     int n_items_cap = 0;
     bool any_large = false;
@@ -444,10 +480,10 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
             seg_beg[8] = (int)items.size();
         }
         n_items_cap = (int)items.size();
-        CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
+        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
         const int n_exact = (int)exact_q.size();
         if (!items.empty())
-            HIPCHK(hipMemcpyAsync(ctx->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
                                   hipMemcpyHostToDevice, st));
         if (n_exact) HIPCHK(hipMemcpyAsync(fb, exact_q.data(), sizeof(int) * n_exact, hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(nfb, &n_exact, sizeof(int), hipMemcpyHostToDevice, st));
@@ -455,47 +491,47 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     } else {
         const int64_t nb = (int64_t)dw->S * dw->C;
         n_items_cap = nq / 64 + (int)nb + 1;
-        CHK(ensure(ctx, ctx->b_scr[3], sizeof(irc::ScreenItem) * (size_t)n_items_cap));
-        CHK(ensure(ctx, ctx->b_plan, irc::plan_buf_bytes(nb)));
-        pb = irc::plan_buf_carve(ctx->b_plan.p, nb);
+        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * (size_t)n_items_cap));
+        CHK(ensure(ctx, ctx->bb->b_plan, irc::plan_buf_bytes(nb)));
+        pb = irc::plan_buf_carve(ctx->bb->b_plan.p, nb);
         HIPCHK(irc::launch_plan(st, dw->S, dw->C, p, dw->site_ptr, dw->prim, min_item, IRC_SCREEN_MAXCAND,
-                                IRC_SCREEN_MAXRUNS, pb, ptr<irc::ScreenItem>(ctx->b_scr[3]), fb, d_seg_beg, nfb));
+                                IRC_SCREEN_MAXRUNS, pb, ptr<irc::ScreenItem>(ctx->bb->b_scr[3]), fb, d_seg_beg, nfb));
         any_large = dw->max_ncand > large_ncand;
     }
     const int KP = irc::screen_kp(p.D);
     const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
     const size_t pcols = (size_t)irc::screen_cols_padded(n_cols);   // tiled rows, whole 32-column groups
-    CHK(ensure(ctx, ctx->b_scr[0], eb * pcols * KP));
-    CHK(ensure(ctx, ctx->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
-    CHK(ensure(ctx, ctx->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
+    CHK(ensure(ctx, ctx->bb->b_scr[0], eb * pcols * KP));
+    CHK(ensure(ctx, ctx->bb->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
+    CHK(ensure(ctx, ctx->bb->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
     // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
     const int slots = (kprime > 32 && any_large) ? 64 : kprime;
-    CHK(ensure(ctx, ctx->b_cand_n, sizeof(int) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_cand_col, sizeof(int) * (size_t)slots * nq));
-    CHK(ensure(ctx, ctx->b_cand_d, sizeof(float) * (size_t)slots * nq));
-    CHK(ensure(ctx, ctx->b_cand_eps, sizeof(double) * (size_t)nq));
-    HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->b_scr[0].p, ctx->b_scr[1].p,
-                                   ptr<float>(ctx->b_scr[2]), feat_out, p.DP));
-    float* run_nmax = ptr<float>(ctx->b_scr[2]) + n_cols;
-    HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->b_scr[2]),
+    CHK(ensure(ctx, ctx->bb->b_cand_n, sizeof(int) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_cand_col, sizeof(int) * (size_t)slots * nq));
+    CHK(ensure(ctx, ctx->bb->b_cand_d, sizeof(float) * (size_t)slots * nq));
+    CHK(ensure(ctx, ctx->bb->b_cand_eps, sizeof(double) * (size_t)nq));
+    HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->bb->b_scr[0].p, ctx->bb->b_scr[1].p,
+                                   ptr<float>(ctx->bb->b_scr[2]), feat_out, p.DP));
+    float* run_nmax = ptr<float>(ctx->bb->b_scr[2]) + n_cols;
+    HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->bb->b_scr[2]),
                                 run_nmax));
     irc::ScreenArgs a;
     a.p = p;
     a.run_nmax = run_nmax;
-    a.hi = ctx->b_scr[0].p;
-    a.lo = ctx->b_scr[1].p;
-    a.norm = ptr<float>(ctx->b_scr[2]);
-    a.items = ptr<irc::ScreenItem>(ctx->b_scr[3]);
+    a.hi = ctx->bb->b_scr[0].p;
+    a.lo = ctx->bb->b_scr[1].p;
+    a.norm = ptr<float>(ctx->bb->b_scr[2]);
+    a.items = ptr<irc::ScreenItem>(ctx->bb->b_scr[3]);
     a.n_items = n_items_cap;
     a.kprime = kprime;
     a.large_ncand = large_ncand;
     a.hist = 0;
     a.eta = irc::screen_eta(sp, p.D, KP);
     a.n_q = nq;
-    a.cand_n = ptr<int>(ctx->b_cand_n);
-    a.cand_col = ptr<int>(ctx->b_cand_col);
-    a.cand_d = ptr<float>(ctx->b_cand_d);
-    a.cand_eps = ptr<double>(ctx->b_cand_eps);
+    a.cand_n = ptr<int>(ctx->bb->b_cand_n);
+    a.cand_col = ptr<int>(ctx->bb->b_cand_col);
+    a.cand_d = ptr<float>(ctx->bb->b_cand_d);
+    a.cand_eps = ptr<double>(ctx->bb->b_cand_eps);
     a.fallback = fb;
     a.n_fallback = nfb;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
@@ -514,15 +550,15 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
             const int lo = ioff[(size_t)s * C], hi = s + 1 < S ? ioff[(size_t)(s + 1) * C] : (int)tot;
             for (int j = 0; j <= 8; ++j) segs[(size_t)s * 9 + j] = lo + (int)((int64_t)(hi - lo) * j / 8);
         }
-        CHK(ensure(ctx, ctx->b_siteseg, sizeof(int) * segs.size()));
-        HIPCHK(hipMemcpyAsync(ctx->b_siteseg.p, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice, st));
+        CHK(ensure(ctx, ctx->bb->b_siteseg, sizeof(int) * segs.size()));
+        HIPCHK(hipMemcpyAsync(ctx->bb->b_siteseg.p, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         HIPCHK(hipEventRecord(ctx->ev[4], st));
         for (int s = 0; s < S; ++s) {
             const int n = segs[(size_t)s * 9 + 8] - segs[(size_t)s * 9];
             if (n <= 0) continue;
             const int g = std::min(n, persistent_grid(ctx, ctx->waves_per_cu));
-            HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, g, seg_ctr, ptr<int>(ctx->b_siteseg) + (size_t)s * 9));
+            HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, g, seg_ctr, ptr<int>(ctx->bb->b_siteseg) + (size_t)s * 9));
         }
         HIPCHK(hipEventRecord(ctx->ev[5], st));
     } else {
@@ -530,7 +566,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
         HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
     }
-    int64_t* dstat = ptr<int64_t>(ctx->b_dstat);
+    int64_t* dstat = ptr<int64_t>(ctx->bb->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
     HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, seg_ctr + 20));
     HIPCHK(irc::launch_stat_copy(st, seg_ctr + 20, nullptr, dstat + 3));
@@ -545,7 +581,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
 // After the call's stream has been synchronized: device counters and kernel times into stats.
 irc_status collect_stats(irc_ctx* ctx, bool events) {
     int64_t d[8] = {0};
-    if (ctx->b_dstat.p) HIPCHK(hipMemcpy(d, ctx->b_dstat.p, sizeof(d), hipMemcpyDeviceToHost));
+    if (ctx->bb->b_dstat.p) HIPCHK(hipMemcpy(d, ctx->bb->b_dstat.p, sizeof(d), hipMemcpyDeviceToHost));
     ctx->stats.n_delta_full = d[4];   // batch calls only (the kNN pass zeroes the counters)
     if (ctx->dstat_screen) {
         ctx->stats.n_screen_fallback = d[1];
@@ -657,6 +693,14 @@ irc_status irc_ctx_create(irc_ctx** out, int device) {
             return IRC_E_HIP;
         }
     }
+    // this device's copy of the no-allocation scratch (the device is current)
+    void* sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+        (void)hipGetLastError();
+        irc_ctx_destroy(ctx);
+        return IRC_E_HIP;
+    }
+    ctx->scratch = reinterpret_cast<char*>(sp);
     *out = ctx;
     return IRC_OK;
 }
@@ -665,17 +709,9 @@ void irc_ctx_destroy(irc_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf* all[] = {&ctx->b_F, &ctx->b_colspk, &ctx->b_feat, &ctx->b_key, &ctx->b_ret, &ctx->b_runb,
-                     &ctx->b_runl, &ctx->b_lptr, &ctx->b_lidx, &ctx->b_qcol, &ctx->b_qblk, &ctx->b_qout,
-                     &ctx->b_small, &ctx->b_flags, &ctx->b_knncol, &ctx->b_dk2, &ctx->b_rhoslot,
-                     &ctx->b_rhocol, &ctx->b_unres, &ctx->b_out_delta, &ctx->b_out_nn, &ctx->b_out_knn,
-                     &ctx->b_out_dk, &ctx->b_out_nn32, &ctx->b_out_knn32, &ctx->b_ncand, &ctx->b_cand_n,
-                     &ctx->b_cand_col, &ctx->b_cand_d, &ctx->b_cand_eps, &ctx->b_plan, &ctx->b_dstat,
-                     &ctx->b_drift, &ctx->b_assign, &ctx->b_siteseg, &ctx->b_tune};
-    for (DevBuf* b : all)
+    // the heap set only: the arena set points into the static scratch array
+    for (DevBuf* b : ctx->heap.all())
         if (b->p) (void)hipFree(b->p);
-    for (auto& b : ctx->b_scr)
-        if (b.p) (void)hipFree(b.p);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -688,10 +724,10 @@ const char* irc_last_error(const irc_ctx* ctx) { return ctx ? ctx->err.c_str() :
 irc_status irc_selftest_mfma(irc_ctx* ctx, float* out) {
     if (!ctx || !out) return IRC_E_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    CHK(ensure(ctx, ctx->b_ncand, sizeof(float) * 4096));
+    CHK(ensure(ctx, ctx->bb->b_ncand, sizeof(float) * 4096));
     hipStream_t st = ctx->stream;
-    HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->b_ncand)));
-    HIPCHK(hipMemcpyAsync(out, ctx->b_ncand.p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
+    HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->bb->b_ncand)));
+    HIPCHK(hipMemcpyAsync(out, ctx->bb->b_ncand.p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -716,10 +752,10 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     DevPlan p;
     CHK(upload_plan(ctx, h, p));
     const int k = p.k;
-    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
-    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->bb->b_knncol, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->bb->b_dk2, sizeof(double) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_out_dk, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_out_knn32, sizeof(int) * (size_t)nq * k));
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
@@ -734,12 +770,12 @@ irc_status irc_knn_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc_o
     block_ranges_cands(h.C, h.runl, h.lptr, h.lidx, ranges);
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), p.feat, p.DP, h.n_cols,
-                &ranges, nullptr, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
-    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
-                                  ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
+                &ranges, nullptr, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2)));
+    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), nullptr,
+                                  ptr<float>(ctx->bb->b_out_dk), ptr<int>(ctx->bb->b_out_knn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    HIPCHK(hipMemcpyAsync(dk_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(knn_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dk_out, ctx->bb->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(knn_out, ctx->bb->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.ms_knn = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
     CHK(collect_stats(ctx, true));
@@ -760,19 +796,19 @@ irc_status irc_delta_drift(irc_ctx* ctx, const irc_site_problem* prob, const irc
     CHK(upload_plan(ctx, h, p));
     std::vector<float> rho_perm((size_t)h.n_cols);
     for (int64_t j = 0; j < h.n_cols; ++j) rho_perm[h.inv[j]] = rho_cols[j];
-    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)h.n_cols));
-    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_rhocol, sizeof(float) * (size_t)h.n_cols));
+    CHK(ensure(ctx, ctx->bb->b_out_delta, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_out_nn32, sizeof(int) * (size_t)nq));
     hipStream_t st = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho_perm.data(), sizeof(float) * h.n_cols, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_rhocol.p, rho_perm.data(), sizeof(float) * h.n_cols, hipMemcpyHostToDevice, st));
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->b_rhocol),
-                                   ptr<float>(ctx->b_out_delta), nullptr, ptr<int>(ctx->b_out_nn32)));
+    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->bb->b_rhocol),
+                                   ptr<float>(ctx->bb->b_out_delta), nullptr, ptr<int>(ctx->bb->b_out_nn32)));
     HIPCHK(hipEventRecord(ctx->ev[1], st));
-    HIPCHK(hipMemcpyAsync(dmin_out, ctx->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dmin_out, ctx->bb->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->bb->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     ctx->stats.ms_delta = ctx->stats.ms_total = event_ms(ctx->ev[0], ctx->ev[1]);
     ctx->stats.n_delta_full = nq;
@@ -863,10 +899,10 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
     const int* dspk = b->col_spike;
     if (!dev) {
         // both copies allocated before their addresses are taken
-        CHK(ensure(ctx, ctx->b_F, sizeof(float) * (size_t)n_cols * D));
-        CHK(ensure(ctx, ctx->b_colspk, sizeof(int) * (size_t)n_cols));
-        dF = ptr<float>(ctx->b_F);
-        dspk = ptr<int>(ctx->b_colspk);
+        CHK(ensure(ctx, ctx->bb->b_F, sizeof(float) * (size_t)n_cols * D));
+        CHK(ensure(ctx, ctx->bb->b_colspk, sizeof(int) * (size_t)n_cols));
+        dF = ptr<float>(ctx->bb->b_F);
+        dspk = ptr<int>(ctx->bb->b_colspk);
     }
     float *o_rho, *o_delta, *o_dk = nullptr;
     int64_t* o_nn;
@@ -878,19 +914,19 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         o_knn = out->knn;
         o_dk = out->dk;
     } else {
-        CHK(ensure(ctx, ctx->b_rhoslot, sizeof(float) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_out_nn, sizeof(int64_t) * (size_t)N));
-        o_rho = ptr<float>(ctx->b_rhoslot);
-        o_delta = ptr<float>(ctx->b_out_delta);
-        o_nn = ptr<int64_t>(ctx->b_out_nn);
+        CHK(ensure(ctx, ctx->bb->b_rhoslot, sizeof(float) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_out_delta, sizeof(float) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_out_nn, sizeof(int64_t) * (size_t)N));
+        o_rho = ptr<float>(ctx->bb->b_rhoslot);
+        o_delta = ptr<float>(ctx->bb->b_out_delta);
+        o_nn = ptr<int64_t>(ctx->bb->b_out_nn);
         if (out->knn) {
-            CHK(ensure(ctx, ctx->b_out_knn, sizeof(int64_t) * (size_t)N * k));
-            o_knn = ptr<int64_t>(ctx->b_out_knn);
+            CHK(ensure(ctx, ctx->bb->b_out_knn, sizeof(int64_t) * (size_t)N * k));
+            o_knn = ptr<int64_t>(ctx->bb->b_out_knn);
         }
         if (out->dk) {
-            CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)N));
-            o_dk = ptr<float>(ctx->b_out_dk);
+            CHK(ensure(ctx, ctx->bb->b_out_dk, sizeof(float) * (size_t)N));
+            o_dk = ptr<float>(ctx->bb->b_out_dk);
         }
     }
     // decided after this call's own allocations above, so a reallocation in them is seen too
@@ -904,7 +940,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         // features in, then the captured device phase; no host round trip before the final one
         HIPCHK(hipEventRecord(ctx->ev[0], st));
         if (!dev)
-            HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
         HIPCHK(hipEventRecord(ctx->ev[1], st));
         HIPCHK(hipGraphLaunch(ctx->graph_exec, st));
         replayed = true;
@@ -913,21 +949,21 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         build_links(C, b->link, lptr, lidx);
         // small descriptors in one buffer: site_ptr[S+1], site_np[S], prim[S+1], lim[C+1]
         const size_t n_small = (size_t)(S + 1) + S + (S + 1) + (C + 1);
-        CHK(ensure(ctx, ctx->b_small, sizeof(int64_t) * n_small));
-        CHK(ensure(ctx, ctx->b_lptr, sizeof(int) * lptr.size()));
-        CHK(ensure(ctx, ctx->b_lidx, sizeof(int) * (lidx.size() + 1)));
-        CHK(ensure(ctx, ctx->b_feat, sizeof(float) * (size_t)n_cols * DP));
-        CHK(ensure(ctx, ctx->b_runb, sizeof(int) * (size_t)S * C * 2));
-        CHK(ensure(ctx, ctx->b_runl, sizeof(int) * (size_t)S * C * 2));
-        CHK(ensure(ctx, ctx->b_qcol, sizeof(int) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_qblk, sizeof(int) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_qout, sizeof(int) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_flags, sizeof(int) * 4));
-        CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)N * k));
-        CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)N));
-        CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n_cols));
-        CHK(ensure(ctx, ctx->b_unres, sizeof(int) * ((size_t)N + 1)));
-        int64_t* d_site_ptr = ptr<int64_t>(ctx->b_small);
+        CHK(ensure(ctx, ctx->bb->b_small, sizeof(int64_t) * n_small));
+        CHK(ensure(ctx, ctx->bb->b_lptr, sizeof(int) * lptr.size()));
+        CHK(ensure(ctx, ctx->bb->b_lidx, sizeof(int) * (lidx.size() + 1)));
+        CHK(ensure(ctx, ctx->bb->b_feat, sizeof(float) * (size_t)n_cols * DP));
+        CHK(ensure(ctx, ctx->bb->b_runb, sizeof(int) * (size_t)S * C * 2));
+        CHK(ensure(ctx, ctx->bb->b_runl, sizeof(int) * (size_t)S * C * 2));
+        CHK(ensure(ctx, ctx->bb->b_qcol, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_qblk, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_qout, sizeof(int) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_flags, sizeof(int) * 4));
+        CHK(ensure(ctx, ctx->bb->b_knncol, sizeof(int) * (size_t)N * k));
+        CHK(ensure(ctx, ctx->bb->b_dk2, sizeof(double) * (size_t)N));
+        CHK(ensure(ctx, ctx->bb->b_rhocol, sizeof(float) * (size_t)n_cols));
+        CHK(ensure(ctx, ctx->bb->b_unres, sizeof(int) * ((size_t)N + 1)));
+        int64_t* d_site_ptr = ptr<int64_t>(ctx->bb->b_small);
         int64_t* d_site_np = d_site_ptr + (S + 1);
         int64_t* d_prim = d_site_np + S;
         int64_t* d_lim = d_prim + (S + 1);
@@ -938,43 +974,43 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
             memcpy(&smallv[2 * S + 1], prim.data(), sizeof(int64_t) * (S + 1));
             memcpy(&smallv[3 * S + 2], b->lim_drift, sizeof(int64_t) * (C + 1));
             HIPCHK(hipMemcpyAsync(d_site_ptr, smallv.data(), sizeof(int64_t) * n_small, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->b_lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_lptr.p, lptr.data(), sizeof(int) * lptr.size(), hipMemcpyHostToDevice, st));
             if (!lidx.empty())
-                HIPCHK(hipMemcpyAsync(ctx->b_lidx.p, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(ctx->bb->b_lidx.p, lidx.data(), sizeof(int) * lidx.size(), hipMemcpyHostToDevice, st));
             // the host vectors are released at the end of this scope
             HIPCHK(hipStreamSynchronize(st));
         }
         if (!dev) {
-            HIPCHK(hipMemcpyAsync(ctx->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
-            HIPCHK(hipMemcpyAsync(ctx->b_colspk.p, b->col_spike, sizeof(int) * (size_t)n_cols, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_F.p, b->F, sizeof(float) * (size_t)n_cols * D, hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_colspk.p, b->col_spike, sizeof(int) * (size_t)n_cols, hipMemcpyHostToDevice, st));
         }
         DevPlan p;
-        p.feat = ptr<float>(ctx->b_feat);
+        p.feat = ptr<float>(ctx->bb->b_feat);
         p.DP = DP;
         p.D = D;
         p.C = C;
         p.k = k;
         p.col_key = dspk;
         p.col_ret = dspk;
-        p.run_beg = ptr<int>(ctx->b_runb);
-        p.run_len = ptr<int>(ctx->b_runl);
-        p.link_ptr = ptr<int>(ctx->b_lptr);
-        p.link_idx = ptr<int>(ctx->b_lidx);
-        p.q_col = ptr<int>(ctx->b_qcol);
-        p.q_blk = ptr<int>(ctx->b_qblk);
-        p.q_out = ptr<int>(ctx->b_qout);
+        p.run_beg = ptr<int>(ctx->bb->b_runb);
+        p.run_len = ptr<int>(ctx->bb->b_runl);
+        p.link_ptr = ptr<int>(ctx->bb->b_lptr);
+        p.link_idx = ptr<int>(ctx->bb->b_lidx);
+        p.q_col = ptr<int>(ctx->bb->b_qcol);
+        p.q_blk = ptr<int>(ctx->bb->b_qblk);
+        p.q_out = ptr<int>(ctx->bb->b_qout);
         p.nq = (int)N;
         p.n_blocks = S * C;
-        int* d_flags = ptr<int>(ctx->b_flags);
-        int* d_nunres = ptr<int>(ctx->b_unres) + N;
+        int* d_flags = ptr<int>(ctx->bb->b_flags);
+        int* d_nunres = ptr<int>(ctx->bb->b_unres) + N;
         const int nq = (int)N;
 
         HIPCHK(irc::launch_fill_int(st, d_flags, 3, 0));
         HIPCHK(hipEventRecord(ctx->ev[0], st));
-        HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->b_runb),
-                                      ptr<int>(ctx->b_runl)));
+        HIPCHK(irc::launch_block_runs(st, S, C, d_site_ptr, d_site_np, d_lim, dspk, ptr<int>(ctx->bb->b_runb),
+                                      ptr<int>(ctx->bb->b_runl)));
         HIPCHK(irc::launch_batch_queries(st, N, S, C, d_site_ptr, d_site_np, d_prim, d_lim, dspk, n_cols,
-                                         ptr<int>(ctx->b_qcol), ptr<int>(ctx->b_qblk), ptr<int>(ctx->b_qout), d_flags));
+                                         ptr<int>(ctx->bb->b_qcol), ptr<int>(ctx->bb->b_qblk), ptr<int>(ctx->bb->b_qout), d_flags));
         HIPCHK(irc::launch_max_ncand(st, S, C, p.link_ptr, p.link_idx, p.run_len, d_flags + 2));
         int hflags[3] = {0, 0, 0};
         HIPCHK(hipMemcpyAsync(hflags, d_flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
@@ -997,16 +1033,16 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
             HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
             CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols,
-                        nullptr, &dw, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), ptr<float>(ctx->b_feat)));
-            HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), o_rho,
+                        nullptr, &dw, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), ptr<float>(ctx->bb->b_feat)));
+            HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), o_rho,
                                           o_dk, nullptr, o_knn));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
-            HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->b_rhocol)));
-            HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_rhocol),
-                                              o_delta, o_nn, ptr<int>(ctx->b_unres), d_nunres));
-            HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
-                                           ptr<float>(ctx->b_rhocol), o_delta, o_nn, nullptr));
-            HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->b_dstat) + 4));
+            HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->bb->b_rhocol)));
+            HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->bb->b_knncol), ptr<float>(ctx->bb->b_rhocol),
+                                              o_delta, o_nn, ptr<int>(ctx->bb->b_unres), d_nunres));
+            HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->bb->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
+                                           ptr<float>(ctx->bb->b_rhocol), o_delta, o_nn, nullptr));
+            HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->bb->b_dstat) + 4));
             return IRC_OK;
         };
         if (graph && ctx->graph_exec && ctx->graph_key == h && ctx->graph_gen == ctx->buf_gen) {
@@ -1073,6 +1109,23 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
 }
 
 // ---------------------------------------------------------------------------
+irc_status irc_rho_delta_batch_noalloc(irc_ctx* ctx, const irc_batch* b, const irc_options* opt, irc_outputs* out) {
+    if (!ctx) return IRC_E_ARG;
+    if (!ctx->scratch) return fail(ctx, IRC_E_HIP, "the scratch array is not mapped on this device");
+    if (opt && (opt->flags & (IRC_FLAG_GRAPH | IRC_FLAG_AUTOTUNE | IRC_FLAG_SITE_LAUNCHES)))
+        return fail(ctx, IRC_E_ARG, "irc_rho_delta_batch_noalloc takes no graph, autotune or per-site launch flag");
+    // the same call with every device buffer carved from the scratch array: a fresh set per call
+    ctx->arena = Bufs{};
+    ctx->scratch_used = 0;
+    ctx->arena_on = true;
+    ctx->bb = &ctx->arena;
+    ctx->desc_key = 0;   // the heap set's planning buffers are not the ones this call writes
+    const irc_status st = irc_rho_delta_batch(ctx, b, opt, out);
+    ctx->bb = &ctx->heap;
+    ctx->arena_on = false;
+    return st;
+}
+
 irc_status irc_autotune(irc_ctx* ctx, const irc_batch* b, const irc_options* base, irc_outputs* out,
                         irc_options* best, irc_tune_report* rep) {
     if (!ctx) return IRC_E_ARG;
@@ -1120,9 +1173,9 @@ irc_status irc_autotune(irc_ctx* ctx, const irc_batch* b, const irc_options* bas
     CHK(run(o0, ms0));
     hipStream_t st = ctx->stream;
     if (dev) {
-        CHK(ensure(ctx, ctx->b_tune, off[5] + 256));
+        CHK(ensure(ctx, ctx->bb->b_tune, off[5] + 256));
         for (int i = 0; i < 5; ++i)
-            if (sz[i]) HIPCHK(hipMemcpyAsync((char*)ctx->b_tune.p + off[i], optr[i], sz[i], hipMemcpyDeviceToDevice, st));
+            if (sz[i]) HIPCHK(hipMemcpyAsync((char*)ctx->bb->b_tune.p + off[i], optr[i], sz[i], hipMemcpyDeviceToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     } else {
         host_ref.resize(off[5]);
@@ -1132,11 +1185,11 @@ irc_status irc_autotune(irc_ctx* ctx, const irc_batch* b, const irc_options* bas
     auto same = [&](bool& eq) -> irc_status {
         eq = true;
         if (dev) {
-            unsigned long long* cnt = reinterpret_cast<unsigned long long*>((char*)ctx->b_tune.p + off[5]);
+            unsigned long long* cnt = reinterpret_cast<unsigned long long*>((char*)ctx->bb->b_tune.p + off[5]);
             HIPCHK(irc::launch_zero_i64(st, reinterpret_cast<int64_t*>(cnt), 1));
             for (int i = 0; i < 5; ++i)
                 if (sz[i])
-                    HIPCHK(irc::launch_count_diff(st, reinterpret_cast<const uint32_t*>((char*)ctx->b_tune.p + off[i]),
+                    HIPCHK(irc::launch_count_diff(st, reinterpret_cast<const uint32_t*>((char*)ctx->bb->b_tune.p + off[i]),
                                                   reinterpret_cast<const uint32_t*>(optr[i]), (int64_t)(sz[i] / 4), cnt));
             unsigned long long nd = 0;
             HIPCHK(hipMemcpyAsync(&nd, cnt, sizeof(nd), hipMemcpyDeviceToHost, st));
@@ -1186,7 +1239,7 @@ irc_status irc_autotune(irc_ctx* ctx, const irc_batch* b, const irc_options* bas
     // leave base's outputs in place
     if (dev) {
         for (int i = 0; i < 5; ++i)
-            if (sz[i]) HIPCHK(hipMemcpyAsync(optr[i], (char*)ctx->b_tune.p + off[i], sz[i], hipMemcpyDeviceToDevice, st));
+            if (sz[i]) HIPCHK(hipMemcpyAsync(optr[i], (char*)ctx->bb->b_tune.p + off[i], sz[i], hipMemcpyDeviceToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     } else {
         for (int i = 0; i < 5; ++i)
@@ -1243,25 +1296,25 @@ irc_status knn_ab_impl(irc_ctx* ctx, int32_t D, int64_t nB, const float* B, cons
     DevPlan p;
     CHK(upload_plan(ctx, h, p));
     const int nq = (int)nA;
-    CHK(ensure(ctx, ctx->b_knncol, sizeof(int) * (size_t)nq * k));
-    CHK(ensure(ctx, ctx->b_dk2, sizeof(double) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_dk, sizeof(float) * (size_t)nq * (K_all ? k : 1)));
-    CHK(ensure(ctx, ctx->b_out_knn32, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->bb->b_knncol, sizeof(int) * (size_t)nq * k));
+    CHK(ensure(ctx, ctx->bb->b_dk2, sizeof(double) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_out_dk, sizeof(float) * (size_t)nq * (K_all ? k : 1)));
+    CHK(ensure(ctx, ctx->bb->b_out_knn32, sizeof(int) * (size_t)nq * k));
     hipStream_t st = ctx->stream;
     ctx->stats = irc_stats{};
     ctx->stats.n_queries = nq;
     const std::vector<QRange> no_ranges;
     CHK(run_knn(ctx, p, nq, IRC_PREC_EXACT, 0, 0, p.feat, p.DP, h.n_cols, &no_ranges, nullptr,
-                ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2)));
-    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->b_knncol), ptr<double>(ctx->b_dk2), nullptr,
-                                  K_all ? nullptr : ptr<float>(ctx->b_out_dk), ptr<int>(ctx->b_out_knn32)));
+                ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2)));
+    HIPCHK(irc::launch_knn_finish(st, p, nq, 1, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), nullptr,
+                                  K_all ? nullptr : ptr<float>(ctx->bb->b_out_dk), ptr<int>(ctx->bb->b_out_knn32)));
     if (K_all) {
-        HIPCHK(irc::launch_knn_dists(st, p, nq, ptr<int>(ctx->b_knncol), ptr<float>(ctx->b_out_dk)));
-        HIPCHK(hipMemcpyAsync(K_all, ctx->b_out_dk.p, sizeof(float) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+        HIPCHK(irc::launch_knn_dists(st, p, nq, ptr<int>(ctx->bb->b_knncol), ptr<float>(ctx->bb->b_out_dk)));
+        HIPCHK(hipMemcpyAsync(K_all, ctx->bb->b_out_dk.p, sizeof(float) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     } else {
-        HIPCHK(hipMemcpyAsync(K_out, ctx->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(K_out, ctx->bb->b_out_dk.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
     }
-    if (I_out) HIPCHK(hipMemcpyAsync(I_out, ctx->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
+    if (I_out) HIPCHK(hipMemcpyAsync(I_out, ctx->bb->b_out_knn32.p, sizeof(int) * (size_t)nq * k, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -1304,15 +1357,15 @@ irc_status irc_delta_knn_ab(irc_ctx* ctx, int32_t D, int64_t nB, const float* B,
     std::vector<float> rho((size_t)(nB + nA));
     memcpy(rho.data(), RB, sizeof(float) * nB);
     memcpy(rho.data() + nB, RA, sizeof(float) * nA);
-    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * rho.size()));
-    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)nq));
-    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_rhocol, sizeof(float) * rho.size()));
+    CHK(ensure(ctx, ctx->bb->b_out_delta, sizeof(float) * (size_t)nq));
+    CHK(ensure(ctx, ctx->bb->b_out_nn32, sizeof(int) * (size_t)nq));
     hipStream_t st = ctx->stream;
-    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho.data(), sizeof(float) * rho.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->b_rhocol),
-                                   ptr<float>(ctx->b_out_delta), nullptr, ptr<int>(ctx->b_out_nn32)));
-    HIPCHK(hipMemcpyAsync(D_out, ctx->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(N_out, ctx->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_rhocol.p, rho.data(), sizeof(float) * rho.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_delta_exact(st, p, nullptr, nq, nullptr, persistent_grid(ctx, 16), 1, ptr<float>(ctx->bb->b_rhocol),
+                                   ptr<float>(ctx->bb->b_out_delta), nullptr, ptr<int>(ctx->bb->b_out_nn32)));
+    HIPCHK(hipMemcpyAsync(D_out, ctx->bb->b_out_delta.p, sizeof(float) * nq, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(N_out, ctx->bb->b_out_nn32.p, sizeof(int) * nq, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -1327,17 +1380,17 @@ irc_status irc_rho_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, co
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const size_t fb = sizeof(float) * (size_t)n12 * D;
-    CHK(ensure(ctx, ctx->b_F, 2 * fb));
-    CHK(ensure(ctx, ctx->b_key, sizeof(int) * 2 * (size_t)n12));
-    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)n1));
-    float* d_f1 = ptr<float>(ctx->b_F);
+    CHK(ensure(ctx, ctx->bb->b_F, 2 * fb));
+    CHK(ensure(ctx, ctx->bb->b_key, sizeof(int) * 2 * (size_t)n12));
+    CHK(ensure(ctx, ctx->bb->b_out_delta, sizeof(float) * (size_t)n1));
+    float* d_f1 = ptr<float>(ctx->bb->b_F);
     float* d_f2 = d_f1 + (size_t)n12 * D;
     HIPCHK(hipMemcpyAsync(d_f1, fet1, fb, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_f2, fet2, fb, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
-    HIPCHK(irc::launch_rho_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->b_key), d_f1, d_f2, dc2,
-                              ptr<float>(ctx->b_out_delta), persistent_grid(ctx, 16)));
-    HIPCHK(hipMemcpyAsync(rho_out, ctx->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_rho_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->bb->b_key), d_f1, d_f2, dc2,
+                              ptr<float>(ctx->bb->b_out_delta), persistent_grid(ctx, 16)));
+    HIPCHK(hipMemcpyAsync(rho_out, ctx->bb->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -1354,22 +1407,22 @@ irc_status irc_delta_drift_dc(irc_ctx* ctx, int32_t D, int64_t n1, int64_t n12, 
     HIPCHK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const size_t fb = sizeof(float) * (size_t)n12 * D;
-    CHK(ensure(ctx, ctx->b_F, 2 * fb));
-    CHK(ensure(ctx, ctx->b_key, sizeof(int) * 2 * (size_t)n12));
-    CHK(ensure(ctx, ctx->b_rhocol, sizeof(float) * (size_t)n12));
-    CHK(ensure(ctx, ctx->b_out_delta, sizeof(float) * (size_t)n1));
-    CHK(ensure(ctx, ctx->b_out_nn32, sizeof(int) * (size_t)n1));
-    float* d_f1 = ptr<float>(ctx->b_F);
+    CHK(ensure(ctx, ctx->bb->b_F, 2 * fb));
+    CHK(ensure(ctx, ctx->bb->b_key, sizeof(int) * 2 * (size_t)n12));
+    CHK(ensure(ctx, ctx->bb->b_rhocol, sizeof(float) * (size_t)n12));
+    CHK(ensure(ctx, ctx->bb->b_out_delta, sizeof(float) * (size_t)n1));
+    CHK(ensure(ctx, ctx->bb->b_out_nn32, sizeof(int) * (size_t)n1));
+    float* d_f1 = ptr<float>(ctx->bb->b_F);
     float* d_f2 = d_f1 + (size_t)n12 * D;
     HIPCHK(hipMemcpyAsync(d_f1, fet1, fb, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d_f2, fet2, fb, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->b_rhocol.p, rho12, sizeof(float) * n12, hipMemcpyHostToDevice, st));
-    HIPCHK(irc::launch_delta_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->b_key), d_f1, d_f2, ptr<float>(ctx->b_rhocol),
-                                dc2, ptr<float>(ctx->b_out_delta), ptr<unsigned>(ctx->b_out_nn32),
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_key.p, miSite12, sizeof(int) * 2 * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->bb->b_rhocol.p, rho12, sizeof(float) * n12, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_delta_dc(st, D, (int)n1, (int)n12, ptr<int>(ctx->bb->b_key), d_f1, d_f2, ptr<float>(ctx->bb->b_rhocol),
+                                dc2, ptr<float>(ctx->bb->b_out_delta), ptr<unsigned>(ctx->bb->b_out_nn32),
                                 persistent_grid(ctx, 16)));
-    HIPCHK(hipMemcpyAsync(delta_out, ctx->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->b_out_nn32.p, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(delta_out, ctx->bb->b_out_delta.p, sizeof(float) * n1, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nneigh_out, ctx->bb->b_out_nn32.p, sizeof(uint32_t) * n1, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }
@@ -1412,8 +1465,8 @@ irc_status irc_calc_drift(irc_ctx* ctx, const irc_drift_problem* pr, int32_t* n_
     const size_t o_d2 = (o_cnt + sizeof(int) * (size_t)nb * C + 255) & ~(size_t)255;
     const size_t o_lnk = (o_d2 + sizeof(int64_t) * (size_t)C * C + 255) & ~(size_t)255;
     const size_t total = o_lnk + (size_t)C * C;
-    CHK(ensure(ctx, ctx->b_drift, total));
-    char* base = reinterpret_cast<char*>(ctx->b_drift.p);
+    CHK(ensure(ctx, ctx->bb->b_drift, total));
+    char* base = reinterpret_cast<char*>(ctx->bb->b_drift.p);
     HIPCHK(hipMemcpyAsync(base + o_amp, pr->amp, sizeof(float) * N, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(base + o_pos, pr->pos, sizeof(float) * N, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(base + o_ea, qa.data(), sizeof(float) * qa.size(), hipMemcpyHostToDevice, st));
@@ -1449,8 +1502,8 @@ irc_status irc_assign_chain(irc_ctx* ctx, int64_t n, const int64_t* nneigh, int3
     // scratch: jump targets, and the host form's copies of nneigh and the labels
     const size_t o_nxt = 0, o_lab = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
     const size_t o_nn = (o_lab + sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
-    CHK(ensure(ctx, ctx->b_assign, dev ? o_lab : o_nn + sizeof(int64_t) * (size_t)n));
-    char* base = reinterpret_cast<char*>(ctx->b_assign.p);
+    CHK(ensure(ctx, ctx->bb->b_assign, dev ? o_lab : o_nn + sizeof(int64_t) * (size_t)n));
+    char* base = reinterpret_cast<char*>(ctx->bb->b_assign.p);
     int32_t* d_nxt = reinterpret_cast<int32_t*>(base + o_nxt);
     int32_t* d_lab = dev ? labels : reinterpret_cast<int32_t*>(base + o_lab);
     const int64_t* d_nn = dev ? nneigh : reinterpret_cast<const int64_t*>(base + o_nn);
@@ -1481,8 +1534,8 @@ irc_status irc_isolation_counts(irc_ctx* ctx, int64_t n, int32_t k, const int64_
     const size_t nc = (size_t)n_clu * (size_t)(n_clu + 1);
     const size_t o_cnt = 0, o_clu = (sizeof(uint64_t) * nc + 255) & ~(size_t)255;
     const size_t o_knn = (o_clu + sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
-    CHK(ensure(ctx, ctx->b_assign, dev ? o_clu : o_knn + sizeof(int64_t) * (size_t)n * k));
-    char* base = reinterpret_cast<char*>(ctx->b_assign.p);
+    CHK(ensure(ctx, ctx->bb->b_assign, dev ? o_clu : o_knn + sizeof(int64_t) * (size_t)n * k));
+    char* base = reinterpret_cast<char*>(ctx->bb->b_assign.p);
     unsigned long long* d_cnt = dev ? reinterpret_cast<unsigned long long*>(counts)
                                     : reinterpret_cast<unsigned long long*>(base + o_cnt);
     const int32_t* d_clu = dev ? clu : reinterpret_cast<const int32_t*>(base + o_clu);

@@ -20,7 +20,7 @@ EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
     "irc_knn_ab", "irc_knn_ab_full", "irc_knn_ab_ib", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
-    "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts", "irc_autotune",
+    "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts", "irc_autotune", "irc_rho_delta_batch_noalloc",
 ]
 
 _c_f32p = ctypes.POINTER(ctypes.c_float)
@@ -115,6 +115,9 @@ def load(path=None):
     lib.irc_rho_delta_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
                                         ctypes.POINTER(Outputs)]
     lib.irc_rho_delta_batch.restype = ctypes.c_int
+    lib.irc_rho_delta_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
+                                                ctypes.POINTER(Outputs)]
+    lib.irc_rho_delta_batch_noalloc.restype = ctypes.c_int
     lib.irc_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Batch), ctypes.POINTER(Options),
                                  ctypes.POINTER(Outputs), ctypes.POINTER(Options), ctypes.POINTER(TuneReport)]
     lib.irc_autotune.restype = ctypes.c_int
@@ -322,6 +325,11 @@ class Context:
                   "large_ncand": best.large_ncand}
         return choice, {"ms_base": rep.ms_base, "ms_best": rep.ms_best, "best_trial": rep.best_trial,
                         "trials": trials}
+
+    def rho_delta_batch_noalloc_device(self, dev, out, opts=None):
+        """irc_rho_delta_batch_noalloc on device-resident data (arguments as rho_delta_batch_device)."""
+        b, o, outs, keep = self._device_structs(dev, out, opts)
+        self._check(self.lib.irc_rho_delta_batch_noalloc(self.h, ctypes.byref(b), ctypes.byref(o), ctypes.byref(outs)))
 
     def rho_delta_batch_device(self, dev, out, opts=None):
         """Device-resident form.  dev: dict of torch tensors on the context's device
