@@ -18,8 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "spikes/s via kNN+rho+delta at 10M spikes; BF16-screen vs FP32, exact kNN"
-PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "exact": 78.6e12}
-PREC = {"exact": 0, "fp32": 1, "bf16": 2}
+PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp8": 5.0e15, "exact": 78.6e12}
+PREC = {"exact": 0, "fp32": 1, "bf16": 2, "fp8": 4}
 
 
 def parse():
@@ -167,11 +167,12 @@ def main():
     line = {
         "metric": METRIC, "value": hf["spikes_per_s"], "unit": "spikes/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": hf["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"bf16": "bf16", "fp32": "fp32", "exact": "f64"}[head],
+        "vs_baseline": None, "dtype": {"bf16": "bf16", "fp32": "fp32", "fp8": "fp8", "exact": "f64"}[head],
         "data": "synthetic: seeded drift recording generated on the GPU, features resident in HBM",
         "config": {"workload": a.workload, "n_spikes": N, "n_sites": ds["n_sites"], "D": D,
                    "n_drift": ds["n_drift"], "k": k, "pairs_per_pass": pairs,
                    "form": {"bf16": "BF16 hi/lo MFMA screen + FP64 re-rank", "fp32": "FP32 MFMA screen + FP64 re-rank",
+                            "fp8": "FP8 e4m3 three-term MFMA screen (per-site scale) + FP64 re-rank",
                             "exact": "FP64 exact"}[head]},
         "roofline": {"bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                      "frac": ach / peak, "traffic": traffic,

@@ -57,7 +57,9 @@ typedef enum {
     IRC_PREC_EXACT = 0,        /* FP64 over every candidate pair */
     IRC_PREC_FP32 = 1,         /* FP32 MFMA distance screen + FP64 re-rank */
     IRC_PREC_BF16_SCREEN = 2,  /* BF16 (hi/lo split) MFMA screen, FP32 accumulate, + FP64 re-rank */
-    IRC_PREC_AUTO = 3          /* BF16 screen where supported, else FP32 */
+    IRC_PREC_AUTO = 3,         /* BF16 screen where supported, else FP32 */
+    IRC_PREC_FP8_SCREEN = 4    /* FP8 e4m3 MFMA screen (three-term split, per-site power-of-two scale),
+                                  FP32 accumulate, + FP64 re-rank */
 } irc_precision;
 
 typedef enum {
@@ -81,7 +83,7 @@ typedef enum {
 typedef struct {
     int32_t precision;  /* irc_precision; default IRC_PREC_AUTO */
     int32_t k_over;     /* over-fetched candidates per query for the screen: 32, 40, 48 or 64
-                           (0 = 48 for the BF16 form, 40 for the FP32 form) */
+                           (0 = 48 for the BF16 form, 40 for the FP32 form, 64 for the FP8 form) */
     int32_t tile;       /* smallest query group screened per wavefront; smaller groups (the
                            leftovers of a block) use the exact kernel (0 = never) */
     int32_t force_fail; /* test hook: non-zero -> return IRC_E_FORCED, outputs untouched */
@@ -244,8 +246,8 @@ typedef struct {
 
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);
 
-/* Self-test of the MFMA lane layouts and the cross-half lane swap the screen kernel relies
- * on (no reference counterpart: a diagnostic).  out: 4096 floats, host memory. */
+/* Self-test of the MFMA lane layouts, the e4m3 decoding of the FP8 MFMA and the cross-half lane swap
+ * the screen kernel relies on (no reference counterpart: a diagnostic).  out: 6144 floats, host memory. */
 irc_status irc_selftest_mfma(irc_ctx* ctx, float* out);
 
 /* ---------------------------------------------------------------------------

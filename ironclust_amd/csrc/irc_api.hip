@@ -40,7 +40,8 @@ struct Bufs {
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
     DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
-    DevBuf b_scr[6];   // screen: [0] hi, [1] lo, [2] norms + run maxima, [3] items, [4] fallback/spill lists, [5] counters
+    DevBuf b_scr[6];   // screen: [0] operand rows, [1] FP8 site scales, [2] norms + run maxima, [3] items,
+                       // [4] fallback/spill lists, [5] counters
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
     DevBuf b_plan;                                         // device-side work planning
     DevBuf b_drift;                                        // drift linkage scratch
@@ -412,7 +413,9 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     HIPCHK(irc::launch_zero_i64(st, ptr<int64_t>(ctx->bb->b_dstat), 8));
     // the screen's operand rows hold D + 2 values in at most 4 MFMA K-blocks of 16 (D <= 62);
     // wider features take the exact path
-    const bool screen = (prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN) && irc::screen_kp(p.D) <= 64;
+    const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : (prec == IRC_PREC_FP8_SCREEN ? irc::SCREEN_FP8 : irc::SCREEN_BF16);
+    const bool screen = (prec == IRC_PREC_FP32 || prec == IRC_PREC_BF16_SCREEN || prec == IRC_PREC_FP8_SCREEN) &&
+                        irc::screen_fits(sp, p.D);
     if (!screen) {
         if (feat_out) HIPCHK(irc::launch_pack_fp32(st, n_cols, p.D, p.DP, F, feat_out));
         ctx->stats.precision_used = IRC_PREC_EXACT;
@@ -424,9 +427,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     }
     ctx->stats.precision_used = prec;
     ctx->dstat_screen = true;
-    const int sp = prec == IRC_PREC_FP32 ? irc::SCREEN_FP32 : irc::SCREEN_BF16;
-    // over-fetch: the BF16 form's looser bound needs a deeper list to certify the k-th
-    int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_BF16 ? 48 : 40);
+    // over-fetch: the looser bounds of the BF16 and FP8 forms need deeper lists to certify the k-th
+    int kprime = k_over > 0 ? k_over : (sp == irc::SCREEN_FP8 ? 64 : (sp == irc::SCREEN_BF16 ? 48 : 40));
     if (kprime < p.k) kprime = p.k;
     kprime = kprime <= 32 ? 32 : (kprime <= 40 ? 40 : (kprime <= 48 ? 48 : 64));
     const int large_ncand = ctx->large_ncand;
@@ -499,10 +501,14 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
         any_large = dw->max_ncand > large_ncand;
     }
     const int KP = irc::screen_kp(p.D);
-    const size_t eb = sp == irc::SCREEN_FP32 ? 4 : 2;
-    const size_t pcols = (size_t)irc::screen_cols_padded(n_cols);   // tiled rows, whole 32-column groups
-    CHK(ensure(ctx, ctx->bb->b_scr[0], eb * pcols * KP));
-    CHK(ensure(ctx, ctx->bb->b_scr[1], sp == irc::SCREEN_FP32 ? 256 : eb * pcols * KP));
+    const int KB = KP / 16;
+    const size_t pcols = (size_t)irc::screen_cols_padded(n_cols);   // whole 32-column groups
+    CHK(ensure(ctx, ctx->bb->b_scr[0], (size_t)16 * irc::screen_np(sp, KB) * pcols));
+    // FP8: the scale of every site ([1]: per segment; the per-site and A/B forms are one segment)
+    const int n_seg = dw ? dw->S : 1;
+    CHK(ensure(ctx, ctx->bb->b_scr[1], sizeof(int64_t) * 2 + sizeof(float) * (size_t)n_seg));
+    const int64_t* seg_ptr = dw ? dw->site_ptr : ptr<int64_t>(ctx->bb->b_scr[1]);
+    float* seg_scale = reinterpret_cast<float*>(ptr<int64_t>(ctx->bb->b_scr[1]) + 2);
     CHK(ensure(ctx, ctx->bb->b_scr[2], sizeof(float) * (size_t)n_cols + sizeof(float) * 2 * (size_t)p.n_blocks));
     // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
     const int slots = (kprime > 32 && any_large) ? 64 : kprime;
@@ -510,23 +516,30 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     CHK(ensure(ctx, ctx->bb->b_cand_col, sizeof(int) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->bb->b_cand_d, sizeof(float) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->bb->b_cand_eps, sizeof(double) * (size_t)nq));
-    HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KP, F, ctx->bb->b_scr[0].p, ctx->bb->b_scr[1].p,
-                                   ptr<float>(ctx->bb->b_scr[2]), feat_out, p.DP));
+    if (sp == irc::SCREEN_FP8) {
+        if (!dw) {
+            const int64_t one[2] = {0, n_cols};
+            HIPCHK(hipMemcpyAsync(ctx->bb->b_scr[1].p, one, sizeof(one), hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));   // `one` leaves scope
+        }
+        HIPCHK(irc::launch_site_scale(st, n_seg, seg_ptr, F, FS, p.D, seg_scale));
+    }
+    HIPCHK(irc::launch_screen_pack(st, sp, n_cols, p.D, FS, KB, F, ctx->bb->b_scr[0].p, ptr<float>(ctx->bb->b_scr[2]),
+                                   feat_out, p.DP, seg_ptr, n_seg, seg_scale));
     float* run_nmax = ptr<float>(ctx->bb->b_scr[2]) + n_cols;
     HIPCHK(irc::launch_run_nmax(st, 2 * (int64_t)p.n_blocks, p.run_beg, p.run_len, ptr<float>(ctx->bb->b_scr[2]),
                                 run_nmax));
     irc::ScreenArgs a;
     a.p = p;
     a.run_nmax = run_nmax;
-    a.hi = ctx->bb->b_scr[0].p;
-    a.lo = ctx->bb->b_scr[1].p;
+    a.rows = ctx->bb->b_scr[0].p;
     a.norm = ptr<float>(ctx->bb->b_scr[2]);
+    a.site_scale = seg_scale;
     a.items = ptr<irc::ScreenItem>(ctx->bb->b_scr[3]);
     a.n_items = n_items_cap;
     a.kprime = kprime;
     a.large_ncand = large_ncand;
-    a.hist = 0;
-    a.eta = irc::screen_eta(sp, p.D, KP);
+    irc::screen_eta(sp, p.D, KP, &a.eta, &a.eta2, &a.eta3);
     a.n_q = nq;
     a.cand_n = ptr<int>(ctx->bb->b_cand_n);
     a.cand_col = ptr<int>(ctx->bb->b_cand_col);
@@ -558,12 +571,12 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
             const int n = segs[(size_t)s * 9 + 8] - segs[(size_t)s * 9];
             if (n <= 0) continue;
             const int g = std::min(n, persistent_grid(ctx, ctx->waves_per_cu));
-            HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, g, seg_ctr, ptr<int>(ctx->bb->b_siteseg) + (size_t)s * 9));
+            HIPCHK(irc::launch_screen_knn_q(st, sp, KB, a, g, seg_ctr, ptr<int>(ctx->bb->b_siteseg) + (size_t)s * 9));
         }
         HIPCHK(hipEventRecord(ctx->ev[5], st));
     } else {
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[4], st));
-        HIPCHK(irc::launch_screen_knn_q(st, sp, KP, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
+        HIPCHK(irc::launch_screen_knn_q(st, sp, KB, a, persistent_grid(ctx, ctx->waves_per_cu), seg_ctr, d_seg_beg));
         if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[5], st));
     }
     int64_t* dstat = ptr<int64_t>(ctx->bb->b_dstat);
@@ -724,10 +737,10 @@ const char* irc_last_error(const irc_ctx* ctx) { return ctx ? ctx->err.c_str() :
 irc_status irc_selftest_mfma(irc_ctx* ctx, float* out) {
     if (!ctx || !out) return IRC_E_ARG;
     HIPCHK(hipSetDevice(ctx->device));
-    CHK(ensure(ctx, ctx->bb->b_ncand, sizeof(float) * 4096));
+    CHK(ensure(ctx, ctx->bb->b_ncand, sizeof(float) * 6144));
     hipStream_t st = ctx->stream;
     HIPCHK(irc::launch_mfma_layout_probe(st, ptr<float>(ctx->bb->b_ncand)));
-    HIPCHK(hipMemcpyAsync(out, ctx->bb->b_ncand.p, sizeof(float) * 4096, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(out, ctx->bb->b_ncand.p, sizeof(float) * 6144, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return IRC_OK;
 }

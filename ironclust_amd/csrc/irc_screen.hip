@@ -3,24 +3,29 @@
 // The screen replaces the per-thread binned-minimum search of search_min_drift.cu:24-94
 // (and the host sort of its nThreads x nA output, matlab/irc2.m:3784-3787) by:
 //   1. distance tiles as a dense contraction on MFMA: with candidate rows
-//      c~ = [c, 1, |c|^2] and query rows q~ = [-2q, |q|^2, 1], one 32x32 product gives
-//      32 candidates x 32 queries of |q|^2 + |c|^2 - 2 q.c directly;
-//        FP32 form: v_mfma_f32_32x32x2_f32
+//      c~ = [c/s, w, |c|^2/(s^2 w)] and query rows q~ = [-2q/s, |q|^2/(s^2 w), w], one 32x32 product
+//      gives 32 candidates x 32 queries of (|q|^2 + |c|^2 - 2 q.c) / s^2 directly;
+//        FP32 form: v_mfma_f32_32x32x2_f32                               (s = w = 1)
 //        BF16 form: v_mfma_f32_32x32x16_bf16 on a hi/lo split of every operand
-//                   (x = hi + lo + r, |r| <= 2^-16 |x|), products hi*hi + hi*lo + lo*hi,
-//                   accumulated in FP32;
+//                   (x = hi + lo + r, |r| <= 2^-16 |x|), products hi*hi + hi*lo + lo*hi   (s = w = 1)
+//        FP8 form:  v_mfma_scale_f32_32x32x64_f8f6f4 on a three-term e4m3 split
+//                   (x = h + m + l + r, |r| <= 2^-12 |x| + 1.07 * 2^-10), eight products (all but l*l);
+//                   s is a per-site power of two (max |x| / s in (16, 32]), w = 256
+//      all accumulated in FP32;
 //   2. a per-query list of the K' smallest screened keys (32-bit keys: truncated
 //      distance bits above the candidate's stream position), kept sorted in VGPRs and
-//      refreshed from an LDS append buffer with bitonic networks;
-//   3. a rigorous bound |screened - exact| <= eta * (|q|^2 + max|c|^2) certifies that
-//      every candidate outside the list is farther than the k-th; the candidates that
-//      can still be in the top k are re-ranked with the canonical FP64 distance and
-//      ordered by (d2, spike id).  Uncertified queries go to the exact kernel.
-// A wavefront owns 64 queries of one (site, chunk) block: two 32-query column tiles
-// share each candidate tile; a lane permutation (v_permlane32_swap) gives every lane
-// all 32 distances of its own query.
+//      refreshed from an LDS append buffer with sorting networks;
+//   3. a rigorous bound on |screened - exact| certifies that every candidate outside the list is
+//      farther than the k-th; the candidates that can still be in the top k are re-ranked with the
+//      canonical FP64 distance and ordered by (d2, spike id).  Uncertified queries go to the exact
+//      kernel.
+// A wavefront owns 64 queries of one (site, chunk) block: two 32-query column tiles share each
+// candidate tile; a lane permutation (v_permlane32_swap) gives every lane all 32 distances of its
+// own query.  The MFMAs of the next candidate tile are issued before the current tile's distances
+// are selected, so the matrix core and the vector ALU work at the same time within a wavefront.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <type_traits>
 
 #include "irc_kernels.h"
@@ -35,6 +40,7 @@ namespace {
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef short short8 __attribute__((ext_vector_type(8)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ uint16_t f2bf_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -43,19 +49,67 @@ __device__ __forceinline__ uint16_t f2bf_rne(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// A workgroup packs 32 columns: their rows are staged in LDS with coalesced loads, one thread
-// per column forms |x|^2, then one thread per (column, 8-dimension chunk) writes coalesced
-// 16-byte pieces of the candidate-form rows [x, 1, |x|^2, 0...] (bf16 hi/lo split, or fp32).
-// The query-form rows [-2x, |x|^2, 1, 0...] are not stored: the screen derives them from the
-// candidate-form rows and |x|^2 (qform_bf16 / qform_f32).
+// OCP e4m3 (bias 7, 3 mantissa bits, subnormals m * 2^-9, largest finite 448): the value nearest v,
+// ties to even, for |v| <= 448.  v / q and rint(v / q) * q are exact (q a power of two).
+__device__ __forceinline__ float e4m3_round(float v) {
+    const int e = (int)((__float_as_uint(v) >> 23) & 0xff) - 127;
+    const float q = __uint_as_float((uint32_t)((e < -6 ? -9 : e - 3) + 127) << 23);
+    return rintf(v / q) * q;
+}
+// the code of an e4m3 value r (exactly representable)
+__device__ __forceinline__ uint32_t e4m3_code(float r) {
+    const uint32_t u = __float_as_uint(r);
+    const uint32_t s = (u >> 24) & 0x80u;
+    const uint32_t a = u & 0x7fffffffu;
+    if (a == 0) return s;
+    const int e = (int)(a >> 23) - 127;
+    if (e < -6) return s | (uint32_t)(__uint_as_float(a) * 512.0f);   // m * 2^-9, m = 1..7
+    return s | ((uint32_t)(e + 7) << 3) | ((a >> 20) & 7u);
+}
+// the three-term split x = h + m + l + r: term t's code
+__device__ __forceinline__ uint32_t e4m3_term(float x, int t) {
+    const float h = e4m3_round(x);
+    if (t == 0) return e4m3_code(h);
+    const float r1 = x - h;                 // exact
+    const float m = e4m3_round(r1);
+    if (t == 1) return e4m3_code(m);
+    return e4m3_code(e4m3_round(r1 - m));   // r1 - m exact
+}
+// -2 x on e4m3 codes (no overflow for |x| <= 224): one exponent step up for normal codes, twice the
+// code for subnormal ones (m * 2^-9 -> 2m * 2^-9, which is also the code of 2^-6 (1 + (2m-8)/8) for
+// 2m >= 8), then the sign flipped
+__device__ __forceinline__ uint32_t e4m3_neg2x4(uint32_t w) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t b = (w >> (8 * i)) & 0xffu;
+        const uint32_t r = (b & 0x78u) ? b + 8u : ((b & 0x80u) | ((b & 7u) << 1));
+        o |= ((r ^ 0x80u) & 0xffu) << (8 * i);
+    }
+    return o;
+}
+
+// candidate-form value at dimension d of a column: x_d / s below D, w at D, |x|^2 / (s^2 w) at D + 1
+__device__ __forceinline__ float cform(const float* x, int d, int D, float inv_s, float w, float nrm_sw) {
+    return d < D ? x[d] * inv_s : (d == D ? w : (d == D + 1 ? nrm_sw : 0.0f));
+}
+
+// A workgroup packs 32 columns: their rows are staged in LDS with coalesced loads, one thread per
+// column forms |x|^2 (and, for FP8, looks up its site's scale), then one thread per (column, piece)
+// writes the 16-byte pieces of the candidate-form rows in the lane-half layout of irc_screen.h:
+// consecutive threads take consecutive columns of one piece, so 32 threads store 512 contiguous
+// bytes.  The query-form rows are not stored: the screen derives them per item.
 constexpr int PACK_COLS = 32;
 
-__global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, const float* __restrict__ F,
-                              void* __restrict__ hi, void* __restrict__ lo, float* __restrict__ norm,
-                              float* __restrict__ feat, int DP) {
+template <int PREC>
+__global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int FS, int np, const float* __restrict__ F,
+                                                     uint4* __restrict__ rows, float* __restrict__ norm,
+                                                     float* __restrict__ feat, int DP,
+                                                     const int64_t* __restrict__ seg_ptr, int n_seg,
+                                                     const float* __restrict__ seg_scale) {
     __shared__ float srow[PACK_COLS * IRC_DMAX];
     __shared__ float snorm[PACK_COLS];
-    const int nch = KP / 8;
+    __shared__ float sscale[PACK_COLS];
     const int64_t col0 = (int64_t)blockIdx.x * PACK_COLS;
     const int ncol = n_cols - col0 < PACK_COLS ? (int)(n_cols - col0) : PACK_COLS;
     // rows col0 .. col0+ncol-1 of F: (ncol-1)*FS + D contiguous floats
@@ -72,6 +126,18 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
         for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
         snorm[threadIdx.x] = n;
         norm[col0 + threadIdx.x] = n;
+        float s = 1.0f;
+        if (PREC == SCREEN_FP8) {
+            // the segment (site) of the column: largest g with seg_ptr[g] <= col
+            const int64_t col = col0 + threadIdx.x;
+            int lo = 0, hi = n_seg - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (seg_ptr[mid] <= col) lo = mid; else hi = mid - 1;
+            }
+            s = seg_scale[lo];
+        }
+        sscale[threadIdx.x] = s;
     }
     __syncthreads();
     if (feat) {
@@ -85,42 +151,99 @@ __global__ void k_screen_pack(int prec, int64_t n_cols, int D, int FS, int KP, c
                                 d0 + 3 < D ? x[d0 + 3] : 0.0f);
         }
     }
-    const int c = threadIdx.x / nch, j = threadIdx.x - c * nch;
-    if (c >= ncol) return;
-    const int64_t col = col0 + c;
-    const float* x = srow + c * IRC_DMAX;
-    const float n = snorm[c];
-    float cv[8];
+    const int nph = np >> 1;
+    for (int e = threadIdx.x; e < PACK_COLS * np; e += blockDim.x) {
+        const int c = e & 31, hj = e >> 5;
+        if (c >= ncol) continue;
+        const int h = hj / nph, j = hj - h * nph;
+        const int64_t col = col0 + c;
+        const float* x = srow + c * IRC_DMAX;
+        uint4 out;
+        if (PREC == SCREEN_FP32) {
+            const int d0 = 8 * j + 4 * h;
+            const float n = snorm[c];
+            out = make_uint4(__float_as_uint(cform(x, d0, D, 1.0f, 1.0f, n)),
+                             __float_as_uint(cform(x, d0 + 1, D, 1.0f, 1.0f, n)),
+                             __float_as_uint(cform(x, d0 + 2, D, 1.0f, 1.0f, n)),
+                             __float_as_uint(cform(x, d0 + 3, D, 1.0f, 1.0f, n)));
+        } else if (PREC == SCREEN_BF16) {
+            const int kb = j >> 1, t = j & 1, d0 = 16 * kb + 8 * h;
+            const float n = snorm[c];
+            uint32_t w[4];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        const int d = 8 * j + t;
-        cv[t] = d < D ? x[d] : (d == D ? 1.0f : (d == D + 1 ? n : 0.0f));
-    }
-    if (prec == SCREEN_FP32) {
-        // fp32 rows are KP / 4 pieces: this thread's 8 values are pieces 2 j and 2 j + 1
-        float4* h4 = reinterpret_cast<float4*>(hi);
-        const int np = KP / 4;
-        h4[screen_piece(col, 2 * j, np)] = make_float4(cv[0], cv[1], cv[2], cv[3]);
-        h4[screen_piece(col, 2 * j + 1, np)] = make_float4(cv[4], cv[5], cv[6], cv[7]);
-    } else {
-        uint32_t ph[4], pl[4];
+            for (int i = 0; i < 4; ++i) {
+                uint32_t pr = 0;
 #pragma unroll
-        for (int t = 0; t < 8; t += 2) {
-            uint16_t h0 = f2bf_rne(cv[t]), h1 = f2bf_rne(cv[t + 1]);
-            uint16_t l0 = f2bf_rne(cv[t] - bf2f(h0)), l1 = f2bf_rne(cv[t + 1] - bf2f(h1));
-            ph[t / 2] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-            pl[t / 2] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+                for (int hh = 0; hh < 2; ++hh) {
+                    const float v = cform(x, d0 + 2 * i + hh, D, 1.0f, 1.0f, n);
+                    const uint16_t hi = f2bf_rne(v);
+                    const uint16_t b = t == 0 ? hi : f2bf_rne(v - bf2f(hi));
+                    pr |= (uint32_t)b << (16 * hh);
+                }
+                w[i] = pr;
+            }
+            out = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            const int t = j >> 1, d0 = 32 * h + 16 * (j & 1);
+            const float s = sscale[c], inv = 1.0f / s;   // powers of two: exact
+            const float nsw = snorm[c] * (inv * inv) * (1.0f / 256.0f);
+            uint32_t w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                uint32_t pr = 0;
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb)
+                    pr |= e4m3_term(cform(x, d0 + 4 * i + bb, D, inv, 256.0f, nsw), t) << (8 * bb);
+                w[i] = pr;
+            }
+            out = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        // bf16 rows are KP / 8 pieces: this thread's 8 values are piece j
-        const int64_t o = screen_piece(col, j, KP / 8);
-        reinterpret_cast<uint4*>(hi)[o] = make_uint4(ph[0], ph[1], ph[2], ph[3]);
-        reinterpret_cast<uint4*>(lo)[o] = make_uint4(pl[0], pl[1], pl[2], pl[3]);
+        rows[screen_off(col, h, np) + 32 * j] = out;
     }
 }
 
-// Query-form piece [-2x, |x|^2, 1, 0...] of dimensions d0 .. d0+7 from the candidate-form piece
-// [x, 1, |x|^2, 0...] of the same column: below D every part is scaled by -2 (exact in bf16),
-// at D and D+1 the parts of |x|^2 and of 1 are put in (at_D, at_D1).
+// One workgroup per segment (site): s = the smallest power of two with max |x| <= 32 s.
+__global__ __launch_bounds__(256) void k_site_scale(const int64_t* __restrict__ seg_ptr, const float* __restrict__ F,
+                                                    int FS, int D, float* __restrict__ seg_scale) {
+    __shared__ float red[4];
+    const int g = blockIdx.x;
+    const int64_t c0 = seg_ptr[g], c1 = seg_ptr[g + 1];
+    float m = 0.0f;
+    for (int64_t e = c0 * D + threadIdx.x; e < c1 * D; e += blockDim.x) {
+        const int64_t c = e / D;
+        const int d = (int)(e - c * D);
+        m = fmaxf(m, fabsf(F[c * FS + d]));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float s = 1.0f;
+        if (m > 0.0f) {
+            int e;
+            const float fr = frexpf(m * (1.0f / 32.0f), &e);   // m / 32 = fr * 2^e, fr in [0.5, 1)
+            s = ldexpf(1.0f, fr == 0.5f ? e - 1 : e);
+        }
+        seg_scale[g] = s;
+    }
+}
+
+// Query-form pieces from the candidate-form pieces of the query columns: below D every value is
+// scaled by -2 (exact in each form), at D and D+1 the query's norm term and w are put in.
+// FP32 piece j of half h: dimensions 8 j + 4 h ..
+__device__ __forceinline__ float4 qform_f32(float4 v, bool valid, int d0, int D, float nrm) {
+    float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int d = d0 + t;
+        const float r = d < D ? -2.0f * c[t] : (d == D ? nrm : (d == D + 1 ? 1.0f : 0.0f));
+        c[t] = valid ? r : 0.0f;
+    }
+    return make_float4(c[0], c[1], c[2], c[3]);
+}
+// BF16 piece: dimensions d0 .. d0+7 of one part (at_D, at_D1: that part of |q|^2 and of 1)
 __device__ __forceinline__ uint4 qform_bf16(uint4 v, bool valid, int d0, int D, uint16_t at_D, uint16_t at_D1) {
     uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -137,17 +260,23 @@ __device__ __forceinline__ uint4 qform_bf16(uint4 v, bool valid, int d0, int D, 
     }
     return make_uint4(w[0], w[1], w[2], w[3]);
 }
-
-// the same for an fp32 piece (dimensions d0 .. d0+3)
-__device__ __forceinline__ float4 qform_f32(float4 v, bool valid, int d0, int D, float nrm) {
-    float c[4] = {v.x, v.y, v.z, v.w};
+// FP8 piece: dimensions d0 .. d0+15 of one term (at_D, at_D1: codes of that term of |q|^2/(s^2 w) and w)
+__device__ __forceinline__ uint4 qform_fp8(uint4 v, bool valid, int d0, int D, uint32_t at_D, uint32_t at_D1) {
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int d = d0 + t;
-        const float r = d < D ? -2.0f * c[t] : (d == D ? nrm : (d == D + 1 ? 1.0f : 0.0f));
-        c[t] = valid ? r : 0.0f;
+        uint32_t o = e4m3_neg2x4(w[t]);
+#pragma unroll
+        for (int bb = 0; bb < 4; ++bb) {
+            const int d = d0 + 4 * t + bb;
+            if (d >= D) {
+                const uint32_t r = d == D ? at_D : (d == D + 1 ? at_D1 : 0u);
+                o = (o & ~(0xffu << (8 * bb))) | (r << (8 * bb));
+            }
+        }
+        w[t] = valid ? o : 0u;
     }
-    return make_float4(c[0], c[1], c[2], c[3]);
+    return make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // keys are signed: a negative screened distance (a candidate within the error bound of the
@@ -197,21 +326,6 @@ __device__ __forceinline__ int ceil_log2(int n) {
     return b;
 }
 
-__device__ __forceinline__ double exact_d2(const float* __restrict__ q, const float* __restrict__ x, int DP) {
-    double acc = 0.0;
-    const float4* q4 = reinterpret_cast<const float4*>(q);
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    for (int d4 = 0; d4 < DP / 4; ++d4) {
-        const float4 a = q4[d4], b = x4[d4];
-        double t;
-        t = (double)a.x - (double)b.x; acc = fma(t, t, acc);
-        t = (double)a.y - (double)b.y; acc = fma(t, t, acc);
-        t = (double)a.z - (double)b.z; acc = fma(t, t, acc);
-        t = (double)a.w - (double)b.w; acc = fma(t, t, acc);
-    }
-    return acc;
-}
-
 // the screened distance a key stands for, negative values read as 0 (a valid lower bound, and
 // non-decreasing along a sorted list); the empty-slot key reads as NaN, which fails every test
 __device__ __forceinline__ float key_dist(int32_t key, uint32_t hmask) {
@@ -221,16 +335,25 @@ __device__ __forceinline__ float key_dist(int32_t key, uint32_t hmask) {
 constexpr int32_t KEY_EMPTY = 0x7fffffff;
 constexpr int32_t KEY_NONE = (int32_t)0x80000000;   // threshold of lanes without a query
 
-template <int PREC, int KB, int KPB, bool HIST>
+// pieces per lane half of the operand rows
+template <int PREC, int KB>
+struct Form {
+    static constexpr int NPH = PREC == SCREEN_FP8 ? 6 : 2 * KB;
+    static constexpr int NP = 2 * NPH;
+};
+
+template <int PREC, int KB, int KPB>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
                                                    const int* __restrict__ seg_beg) {
-    constexpr int KP = 16 * KB;
+    constexpr int NPH = Form<PREC, KB>::NPH;
+    constexpr int NP = Form<PREC, KB>::NP;
     constexpr int LS = KPB <= 32 ? 32 : 64;   // list slots in registers
-    __shared__ uint32_t sbuf[65 * 64];   // rows 0..63: append buffer / histogram / list; row 64: sink
+    __shared__ uint32_t sbuf[65 * 64];   // rows 0..63: append buffer / list; row 64: sink
     __shared__ int s_rb[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_ro[IRC_SCREEN_MAXRUNS + 1];
     __shared__ int s_item;
     const DevPlan& p = a.p;
+    const uint4* rows = reinterpret_cast<const uint4*>(a.rows);
     const int lane = threadIdx.x;
     const int h = lane >> 5, m = lane & 31;
     // workgroups are dispatched round-robin over the 8 XCDs: blockIdx & 7 names the XCD
@@ -297,34 +420,45 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         if (lane == 0) s_ro[nr] = off;
         __syncthreads();
         const int ncand = item.ncand;
+        // FP8: the site's scale (a power of two); distances are screened in units of s^2
+        const float sc = PREC == SCREEN_FP8 ? a.site_scale[s] : 1.0f;
 
-        // ---- query operands (B) for the two 32-query column tiles: query-form rows derived from
-        // the candidate-form rows of the query columns
-        bf16x8 bh[2][KB], bl[2][KB];
-        float4 bq[2][2 * KB];
+        // ---- query operands (B) for the two 32-query column tiles: query-form pieces derived from
+        // the candidate-form pieces of the query columns
+        uint4 Bq[2][NPH];
 #pragma unroll
         for (int T = 0; T < 2; ++T) {
             const int qj = 32 * T + m;
             const bool qv = qj < item.nq;
             const int qc = p.q_col[item.q0 + (qv ? qj : 0)];
             const float qn = a.norm[qc];
-            if constexpr (PREC == SCREEN_BF16) {
-                const uint4* rh = reinterpret_cast<const uint4*>(a.hi);
-                const uint4* rl = reinterpret_cast<const uint4*>(a.lo);
+            const int64_t o = screen_off(qc, h, NP);
+            if constexpr (PREC == SCREEN_FP32) {
+#pragma unroll
+                for (int j = 0; j < NPH; ++j) {
+                    const uint4 v = rows[o + 32 * j];
+                    const float4 f = qform_f32(make_float4(__uint_as_float(v.x), __uint_as_float(v.y),
+                                                           __uint_as_float(v.z), __uint_as_float(v.w)),
+                                               qv, 8 * j + 4 * h, p.D, qn);
+                    Bq[T][j] = make_uint4(__float_as_uint(f.x), __float_as_uint(f.y), __float_as_uint(f.z),
+                                          __float_as_uint(f.w));
+                }
+            } else if constexpr (PREC == SCREEN_BF16) {
                 const uint16_t nh = f2bf_rne(qn), nl = f2bf_rne(qn - bf2f(nh));
 #pragma unroll
-                for (int kb = 0; kb < KB; ++kb) {
-                    const int pc = 2 * kb + h;
-                    const int64_t o = screen_piece(qc, pc, 2 * KB);
-                    bh[T][kb] = __builtin_bit_cast(bf16x8, qform_bf16(rh[o], qv, 8 * pc, p.D, nh, f2bf_rne(1.0f)));
-                    bl[T][kb] = __builtin_bit_cast(bf16x8, qform_bf16(rl[o], qv, 8 * pc, p.D, nl, 0));
+                for (int j = 0; j < NPH; ++j) {
+                    const int t = j & 1;
+                    Bq[T][j] = qform_bf16(rows[o + 32 * j], qv, 16 * (j >> 1) + 8 * h, p.D, t ? nl : nh,
+                                          t ? 0 : f2bf_rne(1.0f));
                 }
             } else {
-                const float4* r = reinterpret_cast<const float4*>(a.hi);
+                const float inv = 1.0f / sc;
+                const float nsw = qn * (inv * inv) * (1.0f / 256.0f);
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) {
-                    const int pc = 2 * g + h;
-                    bq[T][g] = qform_f32(r[screen_piece(qc, pc, 4 * KB)], qv, 4 * pc, p.D, qn);
+                for (int j = 0; j < NPH; ++j) {
+                    const int t = j >> 1;
+                    Bq[T][j] = qform_fp8(rows[o + 32 * j], qv, 32 * h + 16 * (j & 1), p.D, e4m3_term(nsw, t),
+                                         t == 0 ? 0x78u : 0u);   // 0x78: 256
                 }
             }
         }
@@ -333,8 +467,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         const int q_mine = item.q0 + (qmine ? lane : 0);
         const int qc_mine = p.q_col[q_mine];
         const float nq_mine = a.norm[qc_mine];
-        // retire the item's own loads here: the stream loop then waits on operand loads only, so
-        // the compiler can keep the second operand buffer's loads in flight (vmcnt(6), not 0)
+        // retire the item's own loads here: the stream loop then waits on operand loads only
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0), expcnt and lgkmcnt unconstrained
 
         // the item's list depth K' is a compile-time constant: slots past it hold KEY_EMPTY
@@ -345,7 +478,10 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 #pragma unroll
         for (int i = 0; i < LS; ++i) L[i] = KEY_EMPTY;
         int32_t tau = qmine ? KEY_EMPTY : KEY_NONE;
-        int cnt = 0;
+        // the lane's append position as a byte offset into sbuf: row cnt of the lane's column,
+        // wb = 256 cnt + 4 lane
+        const int lane4 = 4 * lane;
+        int wb = lane4;
         const int ib = ceil_log2(ncand > 2 ? ncand : 2);
         const uint32_t pmask = (1u << ib) - 1u, hmask = ~pmask;
         // the position mask in a vector register, so the key is one v_bfi_b32 with the scalar
@@ -354,22 +490,15 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         asm volatile("v_mov_b32 %0, %1" : "=v"(pmask_v) : "s"(pmask));
         int run = 0;
 
-        // the lane's K'-th kept key: the append threshold (L may hold up to LS >= K' keys; those
-        // past K' are larger than the threshold and never change the result)
         auto kth = [&]() -> int32_t { return L[kprime - 1]; };
-        int32_t tau_init = tau;
-        // merge the oldest 32 buffered keys into L, move the rest (at most 32) down
-        // Buffer rows of a lane past its count hold KEY_EMPTY or keys that failed the threshold
-        // when written (larger than L[K'-1] then and ever after, so a merge never keeps them);
-        // rows already merged are cleared.  Without the pre-pass the merge therefore reads the
-        // rows unmasked; with it the threshold can sit below L[K'-1] and the rows are masked.
+        // merge the oldest 32 buffered keys into L, move the rest (at most 32) down.  Buffer rows
+        // of a lane past its count hold KEY_EMPTY or keys that failed the threshold when written
+        // (larger than L[K'-1] then and ever after, so a merge never keeps them); rows already
+        // merged are cleared, so the merge reads the rows unmasked.
         auto compact_half = [&]() {
             int32_t B[32];
 #pragma unroll
-            for (int i = 0; i < 32; ++i) {
-                const int32_t v = (int32_t)sbuf[i * 64 + lane];
-                B[i] = (!HIST || i < cnt) ? v : KEY_EMPTY;
-            }
+            for (int i = 0; i < 32; ++i) B[i] = (int32_t)sbuf[i * 64 + lane];
             oddeven_sort<32>(B);
 #pragma unroll
             for (int i = 0; i < 32; ++i) {
@@ -379,25 +508,20 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             bitonic_clean<LS>(L);
 #pragma unroll
             for (int i = kprime; i < LS; ++i) L[i] = KEY_EMPTY;
-            if (__any(cnt > 32)) {
+            if (__any(wb > lane4 + 32 * 256)) {
 #pragma unroll
                 for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = sbuf[(32 + i) * 64 + lane];
-                if constexpr (!HIST) {
 #pragma unroll
-                    for (int i = 32; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
-                }
-            } else if constexpr (!HIST) {
+                for (int i = 32; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
+            } else {
 #pragma unroll
                 for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
             }
-            cnt = cnt > 32 ? cnt - 32 : 0;
-            const int32_t t = kth();
-            tau = qmine ? (t < tau_init ? t : tau_init) : KEY_NONE;
+            wb = wb > lane4 + 32 * 256 ? wb - 32 * 256 : lane4;
+            tau = qmine ? kth() : KEY_NONE;
         };
 
-        // operand rows of the lane's candidate at stream position pos (bf16: hi then lo halves)
-        constexpr int NA = PREC == SCREEN_BF16 ? 2 * KB : 2 * KB;
-        float4 An0[NA], An1[NA];   // operands of the next two tiles
+        uint4 An0[NPH], An1[NPH];   // operands of the next two tiles
         // tile [pos0, pos0 + 32): `run` is the run holding pos0, with its start column, stream
         // offset and the next run's offset kept wave-uniform (scalar registers); runs that start
         // inside the tile are walked with uniform branches, each lane keeping its column
@@ -408,7 +532,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             run_ro = __builtin_amdgcn_readfirstlane(s_ro[rr]);
             run_next = rr + 1 < nr ? __builtin_amdgcn_readfirstlane(s_ro[rr + 1]) : 0x7fffffff;
         };
-        auto issue = [&](int pos0, float4 (&An)[NA]) {
+        auto issue = [&](int pos0, uint4 (&An)[NPH]) {
             while (pos0 >= run_next) run_enter(run + 1);
             const int pos = pos0 + m;
             int col = run_rb + (pos - run_ro);
@@ -420,85 +544,107 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 }
             }
             if (pos >= ncand) col = run_rb;
-            // tiled layout: a load instruction reads piece p of 32 consecutive columns from
-            // 512 contiguous bytes (two such runs when the tile straddles a 32-column group)
+            // a lane's pieces of one column are 512 bytes apart: one address, immediate offsets
+            const uint4* src = rows + screen_off(col, h, NP);
+#pragma unroll
+            for (int j = 0; j < NPH; ++j) An[j] = src[32 * j];
+        };
+        // the MFMAs of one candidate tile against both query tiles
+        auto mma = [&](const uint4 (&Ac)[NPH], f32x16& c0, f32x16& c1) {
+            c0 = f32x16{};
+            c1 = f32x16{};
             if constexpr (PREC == SCREEN_BF16) {
-                const float4* rh = reinterpret_cast<const float4*>(a.hi);
-                const float4* rl = reinterpret_cast<const float4*>(a.lo);
 #pragma unroll
                 for (int kb = 0; kb < KB; ++kb) {
-                    const int64_t o = screen_piece(col, 2 * kb + h, 2 * KB);
-                    An[2 * kb] = rh[o];
-                    An[2 * kb + 1] = rl[o];
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
+                    const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
+                    const bf16x8 h0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb]);
+                    const bf16x8 h1 = __builtin_bit_cast(bf16x8, Bq[1][2 * kb]);
+                    const bf16x8 l0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb + 1]);
+                    const bf16x8 l1 = __builtin_bit_cast(bf16x8, Bq[1][2 * kb + 1]);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, h0, c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, h1, c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, l0, c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, l1, c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, h0, c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, h1, c1, 0, 0, 0);
+                }
+            } else if constexpr (PREC == SCREEN_FP32) {
+#pragma unroll
+                for (int g = 0; g < NPH; ++g) {
+                    const uint4 av = Ac[g], b0 = Bq[0][g], b1 = Bq[1][g];
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.x), __uint_as_float(b0.x), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.x), __uint_as_float(b1.x), c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.y), __uint_as_float(b0.y), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.y), __uint_as_float(b1.y), c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.z), __uint_as_float(b0.z), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.z), __uint_as_float(b1.z), c1, 0, 0, 0);
+                    c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.w), __uint_as_float(b0.w), c0, 0, 0, 0);
+                    c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.w), __uint_as_float(b1.w), c1, 0, 0, 0);
                 }
             } else {
-                const float4* r = reinterpret_cast<const float4*>(a.hi);
+                auto v8 = [](const uint4& x, const uint4& y) {
+                    return i32x8{(int)x.x, (int)x.y, (int)x.z, (int)x.w, (int)y.x, (int)y.y, (int)y.z, (int)y.w};
+                };
+                const i32x8 Ah = v8(Ac[0], Ac[1]), Am = v8(Ac[2], Ac[3]), Al = v8(Ac[4], Ac[5]);
+#define IRC_F8(A, B, C) C = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 0, 0, 0, 127, 0, 127)
 #pragma unroll
-                for (int g = 0; g < 2 * KB; ++g) An[g] = r[screen_piece(col, 2 * g + h, 4 * KB)];
+                for (int T = 0; T < 2; ++T) {
+                    const i32x8 Bh = v8(Bq[T][0], Bq[T][1]), Bm = v8(Bq[T][2], Bq[T][3]), Bl = v8(Bq[T][4], Bq[T][5]);
+                    f32x16& cc = T == 0 ? c0 : c1;
+                    IRC_F8(Am, Bl, cc);
+                    IRC_F8(Al, Bm, cc);
+                    IRC_F8(Am, Bm, cc);
+                    IRC_F8(Ah, Bl, cc);
+                    IRC_F8(Al, Bh, cc);
+                    IRC_F8(Ah, Bm, cc);
+                    IRC_F8(Am, Bh, cc);
+                    IRC_F8(Ah, Bh, cc);
+                }
+#undef IRC_F8
             }
         };
-        // one pass over the candidate stream: MFMA distance tiles, then consume(v, cpos, valid)
-        // for the 32 distances of the lane's query, then after_tile()
+        // the 32 distances of the lane's query in tile [base, base + 32) from (c0, c1):
+        // lane l < 32 gets query l of tile 0, lane l >= 32 query l-32 of tile 1
+        auto swap_tile = [&](const f32x16& c0, const f32x16& c1, uint32_t (&u)[32]) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(c0[r]), __float_as_uint(c1[r]),
+                                                                false, false);
+                u[2 * r] = sw[0];
+                u[2 * r + 1] = sw[1];
+            }
+        };
+        auto select_tile = [&](const uint32_t (&u)[32], int base, auto&& consume) {
+            auto tile_out = [&](auto full) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
+                        consume((int)u[2 * r + hh], cpos, decltype(full)::value || cpos < ncand);
+                    }
+                }
+            };
+            if (base + 32 <= ncand) tile_out(std::true_type{});
+            else tile_out(std::false_type{});
+        };
+        // one pass over the candidate stream, two tiles per step: the MFMAs of a tile, the loads of
+        // the tile two ahead (into the operand registers just consumed), then the tile's selection.
+        // Two wavefronts per SIMD hide each other's MFMA and load latency.  A tile past the stream
+        // end is multiplied (its rows are valid memory) but not selected.
         auto stream = [&](auto&& consume, auto&& after_tile) {
             run_enter(0);
             issue(0, An0);
             issue(32, An1);
-            // tiles alternate between the two operand buffers; every tile issues its MFMAs and
-            // the loads two tiles ahead unconditionally (positions past the stream read a valid
-            // row), so the count of loads in flight is the same on every path
-            auto tile = [&](int base, float4 (&Ac)[NA]) {
-                f32x16 acc0 = {}, acc1 = {};
-                if constexpr (PREC == SCREEN_BF16) {
-#pragma unroll
-                    for (int kb = 0; kb < KB; ++kb) {
-                        const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
-                        const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[0][kb], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh[1][kb], acc1, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[0][kb], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl[1][kb], acc1, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[0][kb], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh[1][kb], acc1, 0, 0, 0);
-                    }
-                } else {
-#pragma unroll
-                    for (int g = 0; g < 2 * KB; ++g) {
-                        const float4 av = Ac[g];
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[0][g].x, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bq[1][g].x, acc1, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[0][g].y, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bq[1][g].y, acc1, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[0][g].z, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.z, bq[1][g].z, acc1, 0, 0, 0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[0][g].w, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.w, bq[1][g].w, acc1, 0, 0, 0);
-                    }
-                }
-                // the operand registers are free once the MFMAs have read them: the loads of the
-                // tile after next are in flight while this tile and the next are consumed
+            f32x16 c0, c1;
+            auto tile = [&](int base, uint4 (&Ac)[NPH]) {
+                mma(Ac, c0, c1);
                 issue(base + 64, Ac);
                 if (base >= ncand) return;
-                // lane l < 32: query l of tile 0; lane l >= 32: query l-32 of tile 1
                 uint32_t u[32];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]),
-                                                                    __float_as_uint(acc1[r]), false, false);
-                    u[2 * r] = sw[0];
-                    u[2 * r + 1] = sw[1];
-                }
-                auto tile_out = [&](auto full) {
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) {
-#pragma unroll
-                        for (int hh = 0; hh < 2; ++hh) {
-                            const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
-                            consume((int)u[2 * r + hh], cpos, decltype(full)::value || cpos < ncand);
-                        }
-                    }
-                };
-                if (base + 32 <= ncand) tile_out(std::true_type{});
-                else tile_out(std::false_type{});
+                swap_tile(c0, c1, u);
+                select_tile(u, base, consume);
                 after_tile();
             };
             for (int base = 0; base < ncand; base += 64) {
@@ -507,63 +653,30 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             }
         };
 
-        // ---- pass 1 (HIST): per-lane histogram of the screened distances in log buckets (4 per
-        // octave, 16 octaves from |q|^2 * 2^-14) gives a starting threshold that already admits
-        // at least K' candidates, so pass 2 appends about K' keys instead of K'(1 + ln(n/K')).
-        // Row 64 of sbuf absorbs the writes of masked-out elements (no divergent branches).
-        if constexpr (HIST) {
-            if (ncand > kprime) {
-                __syncthreads();
-                uint4* z = reinterpret_cast<uint4*>(sbuf);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) z[i * 64 + lane] = make_uint4(0u, 0u, 0u, 0u);
-                __syncthreads();
-                const int hbase = (int)(__float_as_uint(fmaxf(nq_mine, 1e-30f) * 0x1p-14f) & 0xffe00000u);
-                stream(
-                    [&](int bits, int, bool valid) {
-                        // negative screened values give negative bits: clamped into bucket 0
-                        int b = (bits - hbase) >> 21;
-                        b = b < 0 ? 0 : (b > 63 ? 63 : b);
-                        atomicAdd(&sbuf[(valid ? b : 64) * 64 + lane], 1u);
-                    },
-                    [&]() {});
-                __syncthreads();
-                int cum = 0, tb = 63;
-                for (int b = 0; b < 63; ++b) {
-                    cum += (int)sbuf[b * 64 + lane];
-                    if (cum >= kprime) { tb = b; break; }
-                }
-                if (qmine && tb < 63) tau = hbase + ((tb + 1) << 21);
-                tau_init = tau;
-                __syncthreads();
-            }
-        }
-
-        // ---- pass 2: keys (the screened distance's bits, sign kept, above the candidate's stream
+        // ---- keys (the screened distance's bits, sign kept, above the candidate's stream
         // position) below the threshold go to the lane's append buffer; full buffers merge into L
-        if constexpr (!HIST) {
-            __syncthreads();
+        __syncthreads();
 #pragma unroll
-            for (int i = 0; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
-        }
+        for (int i = 0; i < 64; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
         stream(
             [&](int sbits, int cpos, bool valid) {
                 // key: one bit-field insert (low ib bits from the wave-uniform position, the rest
-                // from the value); slot cnt (< 64: at most 32 buffered when a tile starts) is
-                // scratch until counted; the count takes the compare's carry in one add
+                // from the value); the lane's next buffer row (< 64: at most 32 buffered when a
+                // tile starts) is scratch until counted: the key is stored unconditionally and the
+                // position advances by one row when it passes the threshold.  The compare and the
+                // select do not depend on the position, so each distance adds one dependent add.
                 // (positions past the stream end -- wave-uniform -- write nothing: the buffer rows
                 // past a lane's count must only hold keys that can never enter the list)
                 if (!valid) return;
                 int32_t key;
                 asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(key) : "v"(pmask_v), "s"(cpos), "v"(sbits));
-                sbuf[cnt * 64 + lane] = key;
-                asm("v_cmp_lt_i32 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, 0, %0, vcc"
-                    : "+v"(cnt) : "v"(key), "v"(tau) : "vcc");
+                *reinterpret_cast<int32_t*>(reinterpret_cast<char*>(sbuf) + wb) = key;
+                wb += key < tau ? 256 : 0;
             },
             [&]() {
-                if (__any(cnt > 32)) compact_half();
+                if (__any(wb > lane4 + 32 * 256)) compact_half();
             });
-        while (__any(cnt > 0)) compact_half();
+        while (__any(wb > lane4)) compact_half();
         if (lane == 0) next = claim();
 
         // ---- finalize: certification and the list of candidates the FP64 re-rank must see
@@ -594,15 +707,19 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             const int k = p.k;
             const int nl = ncand < kprime ? ncand : kprime;
             const int kk = ncand < k ? ncand : k;
-            const double eps = a.eta * ((double)nq_mine + (double)nmax_w) + 1e-30;
+            // the bound in real units, then in the screen's units (s^2; powers of two: exact)
+            const double s2 = (double)sc * (double)sc;
+            const double eps = a.eta * ((double)nq_mine + (double)nmax_w) +
+                               a.eta2 * (double)sc * (sqrt((double)nq_mine) + sqrt((double)nmax_w)) +
+                               a.eta3 * s2 + 1e-30;
+            const double eps_s = eps / s2;
             const double qf = ldexp(1.0, -(22 - ib));
             const double dkt = key_dist((int32_t)sbuf[(kk - 1) * 64 + lane], hmask);
-            // every one of the first kk kept candidates has d^2 <= U
-            const double U = dkt * (1.0 + qf) + eps;
-            // every candidate outside L has a key at least min(L[K'-1], starting threshold)
-            const int32_t kl = (int32_t)sbuf[(kprime - 1) * 64 + lane];
-            const int32_t kexcl = kl < tau_init ? kl : tau_init;
-            const bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps > U);
+            // every one of the first kk kept candidates has d^2 <= U (screen units)
+            const double U = dkt * (1.0 + qf) + eps_s;
+            // every candidate outside L has a key at least L[K'-1]
+            const int32_t kexcl = (int32_t)sbuf[(kprime - 1) * 64 + lane];
+            const bool ok = ncand <= kprime || ((double)key_dist(kexcl, hmask) - eps_s > U);
             if (ok) {
                 // the re-rank needs every kept candidate whose lower bound reaches U
                 // (the list is sorted and its screened distances nondecreasing, so the test
@@ -610,13 +727,13 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 int r = kk;
 #pragma unroll
                 for (int j = 0; j < kprime; ++j)
-                    r += (j >= kk && j < nl && (double)key_dist(L[j], hmask) - eps <= U) ? 1 : 0;
+                    r += (j >= kk && j < nl && (double)key_dist(L[j], hmask) - eps_s <= U) ? 1 : 0;
                 for (int j = 0; j < r; ++j) {
                     const int32_t key = (int32_t)sbuf[j * 64 + lane];
                     const int pos = (int)(key & pmask);
                     const int lo = use_tbl ? (int)tbl[pos] : run_of(pos);
                     a.cand_col[(int64_t)j * a.n_q + q_mine] = s_rb[lo] + (pos - s_ro[lo]);
-                    a.cand_d[(int64_t)j * a.n_q + q_mine] = key_dist(key, hmask);
+                    a.cand_d[(int64_t)j * a.n_q + q_mine] = (float)((double)key_dist(key, hmask) * s2);
                 }
                 a.cand_n[q_mine] = r | (kk << 16);
                 a.cand_eps[q_mine] = eps;
@@ -851,12 +968,15 @@ __global__ void k_zero_ints(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-// Lane layout probe of the two MFMA forms and the lane swap the screen relies on.
+// Lane layout probe of the MFMA forms and the lane swap the screen relies on.
 // out[0..1023]   : 32x32x16 bf16 with A[m][k] = (k == 0) * (m + 1), B[k][n] = (k == 0) * 64 * (n + 1)
 //                  -> acc[r] of lane l, stored at out[r * 64 + l]
 // out[1024..2047]: 32x32x2 f32 with the same A, B
 // out[2048..4095]: permlane32_swap(acc0, acc1) of the bf16 result with acc1 = acc0 + 0.5
 //                  (first and second result at out[2048 + r*64 + l], out[3072 + r*64 + l])
+// out[4096..5119]: 32x32x64 fp8 (e4m3, unit scales) with every byte of lane (m, 0) = code 4 m + 1 and of
+//                  lane (n, 0) (B) = code (8 + n) | 0x80 * (n & 1); lanes of half 1 zero
+// out[5120..6143]: the same with the halves exchanged (half 1 carries the codes, half 0 zero)
 __global__ void k_mfma_probe(float* out) {
     const int l = threadIdx.x;
     const int h = l >> 5, m = l & 31;
@@ -873,6 +993,17 @@ __global__ void k_mfma_probe(float* out) {
     f32x16 acc2 = {};
     acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(h == 0 ? (float)(m + 1) : 0.0f, h == 0 ? 64.0f * (m + 1) : 0.0f,
                                                 acc2, 0, 0, 0);
+    f32x16 acc3 = {}, acc4 = {};
+    {
+        const uint32_t ca = (uint32_t)(4 * m + 1), cb = (uint32_t)((8 + m) | (m & 1 ? 0x80 : 0));
+        const int wa = (int)(ca * 0x01010101u), wb = (int)(cb * 0x01010101u);
+        const i32x8 A0 = h == 0 ? i32x8{wa, wa, wa, wa, wa, wa, wa, wa} : i32x8{};
+        const i32x8 B0 = h == 0 ? i32x8{wb, wb, wb, wb, wb, wb, wb, wb} : i32x8{};
+        const i32x8 A1 = h == 1 ? i32x8{wa, wa, wa, wa, wa, wa, wa, wa} : i32x8{};
+        const i32x8 B1 = h == 1 ? i32x8{wb, wb, wb, wb, wb, wb, wb, wb} : i32x8{};
+        acc3 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A0, B0, acc3, 0, 0, 0, 127, 0, 127);
+        acc4 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A1, B1, acc4, 0, 0, 0, 127, 0, 127);
+    }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         out[r * 64 + l] = acc[r];
@@ -881,12 +1012,14 @@ __global__ void k_mfma_probe(float* out) {
                                                         false, false);
         out[2048 + r * 64 + l] = __uint_as_float(sw[0]);
         out[3072 + r * 64 + l] = __uint_as_float(sw[1]);
+        out[4096 + r * 64 + l] = acc3[r];
+        out[5120 + r * 64 + l] = acc4[r];
     }
 }
 
 template <int PREC, int KB>
 hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
-#define IRC_SK(KPB) k_screen_knn<PREC, KB, KPB, false><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg)
+#define IRC_SK(KPB) k_screen_knn<PREC, KB, KPB><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg)
     switch (a.kprime) {
         case 32: IRC_SK(32); break;
         case 40: IRC_SK(40); break;
@@ -902,35 +1035,72 @@ hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr
 
 int screen_kp(int D) { return (D + 2 + 15) / 16 * 16; }
 
-double screen_eta(int prec, int D, int KP) {
-    // unit roundoff taken as 2^-23 (twice FP32's) so the bound also covers truncating accumulation
+void screen_eta(int prec, int D, int KP, double* eta, double* eta2, double* eta3) {
+    // unit roundoff of the FP32 accumulation taken as 2^-23 (twice FP32's) so the bound also covers
+    // truncating accumulation inside the matrix core
     const double u = ldexp(1.0, -23);
     auto gamma = [&](double n) { return n * u / (1.0 - n * u); };
     const double gD = gamma(D);
-    double eta;
+    double e1, e2 = 0.0, e3 = 0.0;
     if (prec == SCREEN_FP32) {
-        eta = 2.0 * (1.0 + gD) * gamma(KP + 2) + gD;
+        e1 = 2.0 * (1.0 + gD) * gamma(KP + 2) + gD;
+    } else if (prec == SCREEN_BF16) {
+        e1 = (3.02 * ldexp(1.0, -16) + 1.02 * gamma(3 * KP + 2)) * 2.0 * (1.0 + gD) + gD;
     } else {
-        eta = (3.02 * ldexp(1.0, -16) + 1.02 * gamma(3 * KP + 2)) * 2.0 * (1.0 + gD) + gD;
+        // Three-term e4m3 split of every scaled operand value a: each term is an e4m3 rounding of
+        // the previous remainder, |v - rnd(v)| <= 2^-4 |v| + 2^-10 (subnormal spacing 2^-9), so the
+        // remainder r after three terms has |r| <= 2^-12 a + 1.07 phi (phi = 2^-10) and the third term
+        // |l| <= 0.00415 a + 2.2 phi.  The eight kept products miss r_q c + q~ r_c + l_q l_c:
+        // per position at most 5.06e-4 a_q a_c + 1.08 phi (a_q + a_c) + 6 phi^2.  Summed over the row
+        // with sum a_q a_c <= 2 (|q|^2 + |c|^2) / s^2 (features, -2x and the norm entries), sum of the
+        // feature a's <= 2 sqrt(D) |x| / s, the norm entries' floors 1.08 phi |x|^2/(256 s^2), and the
+        // exact entry w = 256 meeting the other side's norm term (256 * 1.07 phi each way).  In real
+        // units (times s^2):
+        //   eta  = 2 * 5.06e-4 + 1.08 phi / 256 + accumulation (8 MFMAs of 64 products: gamma(520) * 2)
+        //   eta2 = 1.08 phi * 2 sqrt(D)          (times s (|q| + |c|))
+        //   eta3 = 2 * 256 * 1.07 phi + 64 * 6 phi^2   (times s^2)
+        const double phi = ldexp(1.0, -10);
+        e1 = 2.0 * 5.06e-4 + 1.08 * phi / 256.0 + 2.0 * gamma(8 * 65 + 2);
+        e2 = 1.08 * phi * 2.0 * std::sqrt((double)D);
+        e3 = 2.0 * 256.0 * 1.07 * phi + 64.0 * 6.0 * phi * phi;
     }
-    return 1.25 * eta + 1e-12;
+    *eta = 1.25 * e1 + 1e-12;
+    *eta2 = 1.25 * e2;
+    *eta3 = 1.25 * e3;
 }
 
-hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KP, const float* F,
-                              void* hi, void* lo, float* norm, float* feat, int DP) {
+hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, int FS, int KB, const float* F,
+                              void* rows, float* norm, float* feat, int DP, const int64_t* seg_ptr, int n_seg,
+                              const float* seg_scale) {
     if (feat && FS != D) return hipErrorInvalidValue;
     if (n_cols <= 0) return hipSuccess;
     const int64_t blocks = (n_cols + PACK_COLS - 1) / PACK_COLS;
-    k_screen_pack<<<(unsigned)blocks, PACK_COLS * (KP / 8), 0, st>>>(prec, n_cols, D, FS, KP, F, hi, lo, norm,
-                                                                     feat, DP);
+    const int np = screen_np(prec, KB);
+    uint4* r = reinterpret_cast<uint4*>(rows);
+    if (prec == SCREEN_FP32)
+        k_screen_pack<SCREEN_FP32><<<(unsigned)blocks, 256, 0, st>>>(n_cols, D, FS, np, F, r, norm, feat, DP, seg_ptr,
+                                                                      n_seg, seg_scale);
+    else if (prec == SCREEN_BF16)
+        k_screen_pack<SCREEN_BF16><<<(unsigned)blocks, 256, 0, st>>>(n_cols, D, FS, np, F, r, norm, feat, DP, seg_ptr,
+                                                                      n_seg, seg_scale);
+    else
+        k_screen_pack<SCREEN_FP8><<<(unsigned)blocks, 256, 0, st>>>(n_cols, D, FS, np, F, r, norm, feat, DP, seg_ptr,
+                                                                     n_seg, seg_scale);
     return hipGetLastError();
 }
 
-hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KP, const ScreenArgs& a, int grid, int* seg_ctr,
+hipError_t launch_site_scale(hipStream_t st, int n_seg, const int64_t* seg_ptr, const float* F, int FS, int D,
+                             float* seg_scale) {
+    if (n_seg <= 0) return hipSuccess;
+    k_site_scale<<<n_seg, 256, 0, st>>>(seg_ptr, F, FS, D, seg_scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KB, const ScreenArgs& a, int grid, int* seg_ctr,
                                const int* seg_beg) {
     if (a.n_items <= 0) return hipSuccess;
     k_zero_ints<<<1, 64, 0, st>>>(seg_ctr, 8);
-    const int KB = KP / 16;
+    if (prec == SCREEN_FP8) return launch_kb<SCREEN_FP8, 4>(st, a, grid, seg_ctr, seg_beg);
     if (prec == SCREEN_BF16) {
         switch (KB) {
             case 1: return launch_kb<SCREEN_BF16, 1>(st, a, grid, seg_ctr, seg_beg);

@@ -41,7 +41,7 @@ def _opts(P):
     prec = P.get("precision", libirc.PREC_AUTO)
     if isinstance(prec, str):
         prec = {"exact": libirc.PREC_EXACT, "fp32": libirc.PREC_FP32, "bf16": libirc.PREC_BF16_SCREEN,
-                "auto": libirc.PREC_AUTO}[prec]
+                "fp8": libirc.PREC_FP8_SCREEN, "auto": libirc.PREC_AUTO}[prec]
     return libirc.make_options(precision=prec, k_over=int(P.get("k_over", 0)),
                                force_fail=bool(P.get("force_fail", False)), flags=int(P.get("flags", 0)),
                                large_ncand=int(P.get("large_ncand", 0)))

@@ -12,7 +12,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "_lib", "libirc_hip.so")
 
 IRC_OK, IRC_E_ARG, IRC_E_NOMEM, IRC_E_HIP, IRC_E_UNSUPPORTED, IRC_E_FORCED = range(6)
-PREC_EXACT, PREC_FP32, PREC_BF16_SCREEN, PREC_AUTO = range(4)
+PREC_EXACT, PREC_FP32, PREC_BF16_SCREEN, PREC_AUTO, PREC_FP8_SCREEN = range(5)
 MEM_HOST, MEM_DEVICE = 0, 1
 FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_SITE_LAUNCHES = 4, 8, 16          # irc_options.flags
 
@@ -216,7 +216,7 @@ class Context:
         return {f: getattr(s, f) for f, _ in Stats._fields_}
 
     def selftest_mfma(self):
-        out = np.zeros(4096, np.float32)
+        out = np.zeros(6144, np.float32)
         self._check(self.lib.irc_selftest_mfma(self.h, _p(out, _c_f32p)))
         return out
 

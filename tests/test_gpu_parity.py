@@ -10,7 +10,7 @@ from tests.helpers import int_dataset
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ["exact", "fp32", "bf16"]
+PRECS = ["exact", "fp32", "bf16", "fp8"]
 
 
 def _check_batch(ds, prec, ctxname=""):
@@ -358,7 +358,7 @@ def test_mfma_lane_layout(ctx):
     """The screen relies on the 32x32 MFMA output layout (register r of lane l holds row
     8*(r//4) + 4*(l//32) + r%4, column l%32) and on v_permlane32_swap exchanging the upper
     half of its first operand with the lower half of its second."""
-    out = ctx.selftest_mfma().reshape(4, 16, 64)
+    out = ctx.selftest_mfma().reshape(6, 16, 64)
     r = np.arange(16)[:, None]
     l = np.arange(64)[None, :]
     row = 8 * (r // 4) + 4 * (l // 32) + r % 4
@@ -372,6 +372,17 @@ def test_mfma_lane_layout(ctx):
     second = np.where(l < 32, np.roll(acc0, -32, axis=1), acc1)
     np.testing.assert_array_equal(out[2], first, err_msg="permlane32_swap first result")
     np.testing.assert_array_equal(out[3], second, err_msg="permlane32_swap second result")
+    # 32x32x64 e4m3 (OCP: bias 7, subnormals m * 2^-9): row m's 32 bytes of one lane half all hold
+    # code 4m + 1, column n's code (8 + n) with the sign of odd n; the product sums 32 equal terms
+    def e4m3(c):
+        s = -1.0 if c & 0x80 else 1.0
+        e, mt = (c >> 3) & 15, c & 7
+        return s * (mt * 2.0 ** -9 if e == 0 else (1 + mt / 8) * 2.0 ** (e - 7))
+    xa = np.array([e4m3(4 * i + 1) for i in range(32)])
+    xb = np.array([e4m3((8 + i) | (0x80 if i & 1 else 0)) for i in range(32)])
+    want8 = (32.0 * xa[row] * xb[col]).astype(np.float32)
+    np.testing.assert_array_equal(out[4], want8, err_msg="fp8 32x32x64 layout and e4m3 decoding (half 0)")
+    np.testing.assert_array_equal(out[5], want8, err_msg="fp8 32x32x64 layout and e4m3 decoding (half 1)")
 
 
 def _dev_batch(ds, k):
