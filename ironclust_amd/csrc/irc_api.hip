@@ -434,11 +434,15 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     const int large_ncand = ctx->large_ncand;
     // [0, nq) exact-kernel list, [nq] its count, [nq + 1, 2 nq + 1) the first re-rank pass's overflows
     CHK(ensure(ctx, ctx->bb->b_scr[4], sizeof(int) * (2 * (size_t)nq + 1)));
-    CHK(ensure(ctx, ctx->bb->b_scr[5], sizeof(int) * 32));
+    // [0, 8) unused, [8, 17) the XCD segment offsets, [20] the re-rank's spill count, [32, 57) the
+    // screen launch's counters and split offsets
+    CHK(ensure(ctx, ctx->bb->b_scr[5], sizeof(int) * 64));
     int* fb = ptr<int>(ctx->bb->b_scr[4]);
     int* nfb = fb + nq;
-    int* seg_ctr = ptr<int>(ctx->bb->b_scr[5]);
-    int* d_seg_beg = seg_ctr + 8;
+    int* scr5 = ptr<int>(ctx->bb->b_scr[5]);
+    int* d_seg_beg = scr5 + 8;
+    int* seg_ctr = scr5 + 32;
+    int n_items_slots = 0;   // room of each of the two item arrays in b_scr[3]
     int n_items_cap = 0;
     bool any_large = false;
     irc::PlanBuf pb{};
@@ -482,7 +486,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
             seg_beg[8] = (int)items.size();
         }
         n_items_cap = (int)items.size();
-        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * (items.size() + 1)));
+        n_items_slots = (int)items.size() + 1;
+        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * 2 * (size_t)n_items_slots));
         const int n_exact = (int)exact_q.size();
         if (!items.empty())
             HIPCHK(hipMemcpyAsync(ctx->bb->b_scr[3].p, items.data(), sizeof(irc::ScreenItem) * items.size(),
@@ -493,7 +498,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     } else {
         const int64_t nb = (int64_t)dw->S * dw->C;
         n_items_cap = nq / 64 + (int)nb + 1;
-        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * (size_t)n_items_cap));
+        n_items_slots = n_items_cap;
+        CHK(ensure(ctx, ctx->bb->b_scr[3], sizeof(irc::ScreenItem) * 2 * (size_t)n_items_slots));
         CHK(ensure(ctx, ctx->bb->b_plan, irc::plan_buf_bytes(nb)));
         pb = irc::plan_buf_carve(ctx->bb->b_plan.p, nb);
         HIPCHK(irc::launch_plan(st, dw->S, dw->C, p, dw->site_ptr, dw->prim, min_item, IRC_SCREEN_MAXCAND,
@@ -536,6 +542,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.norm = ptr<float>(ctx->bb->b_scr[2]);
     a.site_scale = seg_scale;
     a.items = ptr<irc::ScreenItem>(ctx->bb->b_scr[3]);
+    a.items_kind = ptr<irc::ScreenItem>(ctx->bb->b_scr[3]) + n_items_slots;
     a.n_items = n_items_cap;
     a.kprime = kprime;
     a.large_ncand = large_ncand;
@@ -581,8 +588,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     }
     int64_t* dstat = ptr<int64_t>(ctx->bb->b_dstat);
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 0));
-    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, seg_ctr + 20));
-    HIPCHK(irc::launch_stat_copy(st, seg_ctr + 20, nullptr, dstat + 3));
+    HIPCHK(irc::launch_screen_rerank(st, a, knn_col, dk2, nfb + 1, scr5 + 20));
+    HIPCHK(irc::launch_stat_copy(st, scr5 + 20, nullptr, dstat + 3));
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));

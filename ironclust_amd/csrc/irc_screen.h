@@ -45,6 +45,8 @@ struct ScreenArgs {
     const float* run_nmax;   // [n_blocks * 2] largest |x|^2 of each (block, part) run
     const float* site_scale; // FP8: power-of-two scale s of each site (block / C), else unused
     const ScreenItem* items;
+    ScreenItem* items_kind;  // [n_items] the items reordered by the launch: per XCD segment, those of
+                             // more than 32 queries, then those of at most 32
     int n_items;
     int kprime;              // over-fetched candidates per query (32, 40, 48 or 64)
     int large_ncand;         // blocks with more candidates keep 64 (when kprime > 32)
@@ -75,8 +77,9 @@ hipError_t launch_screen_pack(hipStream_t st, int prec, int64_t n_cols, int D, i
 // FP8: seg_scale[s] = 2^e, the smallest power of two with max |F| over segment s's columns <= 32 * 2^e
 hipError_t launch_site_scale(hipStream_t st, int n_seg, const int64_t* seg_ptr, const float* F, int FS, int D,
                              float* seg_scale);
-// seg_ctr: 8 device counters (zeroed by the launch); seg_beg: 9 device offsets of the
-// per-XCD item segments (items of segment x are dequeued first by workgroups on XCD x).
+// seg_ctr: 25 ints of device scratch (counters and the split offsets; set by the launch);
+// seg_beg: 9 device offsets of the per-XCD item segments (items of segment x are dequeued first
+// by workgroups on XCD x).  Items of at most 32 queries run in a kernel of their own.
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KB, const ScreenArgs& a, int grid, int* seg_ctr,
                                const int* seg_beg);
 // FP64 re-rank of the certified lists into the k-lists (knn_col, dk2) in two passes: a short

@@ -335,6 +335,18 @@ __device__ __forceinline__ float key_dist(int32_t key, uint32_t hmask) {
 constexpr int32_t KEY_EMPTY = 0x7fffffff;
 constexpr int32_t KEY_NONE = (int32_t)0x80000000;   // threshold of lanes without a query
 
+// Appends `value` of every lane with `pred` to list[0 .. *n) with one atomic per wavefront (the
+// lanes' order within the wavefront is kept).  Called by all 64 lanes.
+__device__ __forceinline__ void wave_append(int* list, int* n, bool pred, int value) {
+    const uint64_t bal = __ballot(pred);
+    if (bal == 0) return;
+    int base = 0;
+    if (threadIdx.x % 64 == (unsigned)__builtin_ctzll(bal)) base = atomicAdd(n, __builtin_popcountll(bal));
+    base = __shfl(base, __builtin_ctzll(bal), 64);
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    if (pred) list[base + rank] = value;
+}
+
 // pieces per lane half of the operand rows
 template <int PREC, int KB>
 struct Form {
@@ -342,9 +354,11 @@ struct Form {
     static constexpr int NP = 2 * NPH;
 };
 
-template <int PREC, int KB, int KPB>
+// HALF: the kernel of the items of at most 32 queries (run in halves, below); the items of
+// segment x are items[seg_lo[x] .. seg_hi[x])
+template <int PREC, int KB, int KPB, bool HALF>
 __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restrict__ seg_ctr,
-                                                   const int* __restrict__ seg_beg) {
+                                                   const int* __restrict__ seg_lo, const int* __restrict__ seg_hi) {
     constexpr int NPH = Form<PREC, KB>::NPH;
     constexpr int NP = Form<PREC, KB>::NP;
     constexpr int LS = KPB <= 32 ? 32 : 64;   // list slots in registers
@@ -364,9 +378,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         int got = -1;
         for (int t = 0; t < 8 && got < 0; ++t) {
             const int sg = (my_seg + t) & 7;
-            if (seg_ctr[sg] >= seg_beg[sg + 1] - seg_beg[sg]) continue;
+            if (seg_ctr[sg] >= seg_hi[sg] - seg_lo[sg]) continue;
             const int idx = atomicAdd(&seg_ctr[sg], 1);
-            if (seg_beg[sg] + idx < seg_beg[sg + 1]) got = seg_beg[sg] + idx;
+            if (seg_lo[sg] + idx < seg_hi[sg]) got = seg_lo[sg] + idx;
         }
         return got;
     };
@@ -379,7 +393,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         __syncthreads();
         const int it = s_item;
         if (it < 0) break;
-        const ScreenItem item = a.items[it];
+        const ScreenItem item = a.items_kind[it];
         const int s = item.blk / p.C, c = item.blk - s * p.C;
         // the block's candidate runs (linked chunks in link order, primary part then
         // secondary), one lane per (link, part): stream offsets by a wave prefix sum
@@ -427,7 +441,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         // the candidate-form pieces of the query columns
         uint4 Bq[2][NPH];
 #pragma unroll
-        for (int T = 0; T < 2; ++T) {
+        for (int T = 0; T < (HALF ? 1 : 2); ++T) {
             const int qj = 32 * T + m;
             const bool qv = qj < item.nq;
             const int qc = p.q_col[item.q0 + (qv ? qj : 0)];
@@ -462,8 +476,13 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 }
             }
         }
-        // after the lane swap, lane l owns query l of the item
+        // after the lane swap, lane l owns query l of the item.  An item of at most 32 queries runs
+        // in halves: both MFMAs of a step take query tile 0, against two consecutive candidate
+        // tiles, so lane l < 32 screens its query over the even 32-candidate tiles and lane l + 32
+        // the same query over the odd ones; the two lists merge before the finalize.
+        constexpr bool half = HALF;
         const bool qmine = lane < item.nq;
+        const bool qact0 = half ? (lane & 31) < item.nq : qmine;
         const int q_mine = item.q0 + (qmine ? lane : 0);
         const int qc_mine = p.q_col[q_mine];
         const float nq_mine = a.norm[qc_mine];
@@ -477,7 +496,8 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
         int32_t L[LS];
 #pragma unroll
         for (int i = 0; i < LS; ++i) L[i] = KEY_EMPTY;
-        int32_t tau = qmine ? KEY_EMPTY : KEY_NONE;
+        bool qact = qact0;   // the lane screens a query (in a half item: half of its stream)
+        int32_t tau = qact ? KEY_EMPTY : KEY_NONE;
         // the lane's append position as a byte offset into sbuf: row cnt of the lane's column,
         // wb = 256 cnt + 4 lane
         const int lane4 = 4 * lane;
@@ -518,7 +538,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 for (int i = 0; i < 32; ++i) sbuf[i * 64 + lane] = (uint32_t)KEY_EMPTY;
             }
             wb = wb > lane4 + 32 * 256 ? wb - 32 * 256 : lane4;
-            tau = qmine ? kth() : KEY_NONE;
+            tau = qact ? kth() : KEY_NONE;
         };
 
         uint4 An0[NPH], An1[NPH];   // operands of the next two tiles
@@ -559,9 +579,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                     const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
                     const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
                     const bf16x8 h0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb]);
-                    const bf16x8 h1 = __builtin_bit_cast(bf16x8, Bq[1][2 * kb]);
+                    const bf16x8 h1 = __builtin_bit_cast(bf16x8, Bq[HALF ? 0 : 1][2 * kb]);
                     const bf16x8 l0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb + 1]);
-                    const bf16x8 l1 = __builtin_bit_cast(bf16x8, Bq[1][2 * kb + 1]);
+                    const bf16x8 l1 = __builtin_bit_cast(bf16x8, Bq[HALF ? 0 : 1][2 * kb + 1]);
                     c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, h0, c0, 0, 0, 0);
                     c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, h1, c1, 0, 0, 0);
                     c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, l0, c0, 0, 0, 0);
@@ -572,7 +592,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             } else if constexpr (PREC == SCREEN_FP32) {
 #pragma unroll
                 for (int g = 0; g < NPH; ++g) {
-                    const uint4 av = Ac[g], b0 = Bq[0][g], b1 = Bq[1][g];
+                    const uint4 av = Ac[g], b0 = Bq[0][g], b1 = Bq[HALF ? 0 : 1][g];
                     c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.x), __uint_as_float(b0.x), c0, 0, 0, 0);
                     c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.x), __uint_as_float(b1.x), c1, 0, 0, 0);
                     c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.y), __uint_as_float(b0.y), c0, 0, 0, 0);
@@ -590,7 +610,8 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 #define IRC_F8(A, B, C) C = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 0, 0, 0, 127, 0, 127)
 #pragma unroll
                 for (int T = 0; T < 2; ++T) {
-                    const i32x8 Bh = v8(Bq[T][0], Bq[T][1]), Bm = v8(Bq[T][2], Bq[T][3]), Bl = v8(Bq[T][4], Bq[T][5]);
+                    const int TB = HALF ? 0 : T;
+                    const i32x8 Bh = v8(Bq[TB][0], Bq[TB][1]), Bm = v8(Bq[TB][2], Bq[TB][3]), Bl = v8(Bq[TB][4], Bq[TB][5]);
                     f32x16& cc = T == 0 ? c0 : c1;
                     IRC_F8(Am, Bl, cc);
                     IRC_F8(Al, Bm, cc);
@@ -601,6 +622,47 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                     IRC_F8(Am, Bh, cc);
                     IRC_F8(Ah, Bh, cc);
                 }
+#undef IRC_F8
+            }
+        };
+        // half items: query tile 0 against one candidate tile
+        auto mma_one = [&](const uint4 (&Ac)[NPH], f32x16& cc) {
+            cc = f32x16{};
+            if constexpr (PREC == SCREEN_BF16) {
+#pragma unroll
+                for (int kb = 0; kb < KB; ++kb) {
+                    const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[2 * kb]);
+                    const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[2 * kb + 1]);
+                    const bf16x8 h0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb]);
+                    const bf16x8 l0 = __builtin_bit_cast(bf16x8, Bq[0][2 * kb + 1]);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, h0, cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, l0, cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, h0, cc, 0, 0, 0);
+                }
+            } else if constexpr (PREC == SCREEN_FP32) {
+#pragma unroll
+                for (int g = 0; g < NPH; ++g) {
+                    const uint4 av = Ac[g], b0 = Bq[0][g];
+                    cc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.x), __uint_as_float(b0.x), cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.y), __uint_as_float(b0.y), cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.z), __uint_as_float(b0.z), cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(av.w), __uint_as_float(b0.w), cc, 0, 0, 0);
+                }
+            } else {
+                auto v8 = [](const uint4& x, const uint4& y) {
+                    return i32x8{(int)x.x, (int)x.y, (int)x.z, (int)x.w, (int)y.x, (int)y.y, (int)y.z, (int)y.w};
+                };
+                const i32x8 Bh = v8(Bq[0][0], Bq[0][1]), Bm = v8(Bq[0][2], Bq[0][3]), Bl = v8(Bq[0][4], Bq[0][5]);
+                const i32x8 Ah = v8(Ac[0], Ac[1]), Am = v8(Ac[2], Ac[3]), Al = v8(Ac[4], Ac[5]);
+#define IRC_F8(A, B, C) C = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B, C, 0, 0, 0, 127, 0, 127)
+                IRC_F8(Am, Bl, cc);
+                IRC_F8(Al, Bm, cc);
+                IRC_F8(Am, Bm, cc);
+                IRC_F8(Ah, Bl, cc);
+                IRC_F8(Al, Bh, cc);
+                IRC_F8(Ah, Bm, cc);
+                IRC_F8(Am, Bh, cc);
+                IRC_F8(Ah, Bh, cc);
 #undef IRC_F8
             }
         };
@@ -629,6 +691,28 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             if (base + 32 <= ncand) tile_out(std::true_type{});
             else tile_out(std::false_type{});
         };
+        // half items: lane half h holds candidates base + 32 h + (0..31); the key carries the
+        // position less 32 h (wave-uniform), put right when the halves merge
+        auto select_half = [&](const uint32_t (&u)[32], int base, auto&& consume) {
+            if (base + 64 <= ncand) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh)
+                        consume((int)u[2 * r + hh], base + 8 * (r >> 2) + 4 * hh + (r & 3), true);
+                }
+            } else {
+                const int lim = ncand - 32 * h;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+#pragma unroll
+                    for (int hh = 0; hh < 2; ++hh) {
+                        const int cpos = base + 8 * (r >> 2) + 4 * hh + (r & 3);
+                        consume((int)u[2 * r + hh], cpos, cpos < lim);
+                    }
+                }
+            }
+        };
         // one pass over the candidate stream, two tiles per step: the MFMAs of a tile, the loads of
         // the tile two ahead (into the operand registers just consumed), then the tile's selection.
         // Two wavefronts per SIMD hide each other's MFMA and load latency.  A tile past the stream
@@ -638,6 +722,19 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             issue(0, An0);
             issue(32, An1);
             f32x16 c0, c1;
+            if constexpr (half) {
+                for (int base = 0; base < ncand; base += 64) {
+                    mma_one(An0, c0);
+                    issue(base + 64, An0);
+                    mma_one(An1, c1);
+                    issue(base + 96, An1);
+                    uint32_t u[32];
+                    swap_tile(c0, c1, u);
+                    select_half(u, base, consume);
+                    after_tile();
+                }
+                return;
+            }
             auto tile = [&](int base, uint4 (&Ac)[NPH]) {
                 mma(Ac, c0, c1);
                 issue(base + 64, Ac);
@@ -676,7 +773,23 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             [&]() {
                 if (__any(wb > lane4 + 32 * 256)) compact_half();
             });
-        while (__any(wb > lane4)) compact_half();
+        // flush the buffers; a half item then hands lane l + 32's list (positions put right) to
+        // lane l, which merges it as kprime buffered keys, and flushes again
+#pragma nounroll
+        for (int pass = 0; pass < (half ? 2 : 1); ++pass) {
+            if (pass == 1) {
+                __syncthreads();
+                if (h == 1) {
+#pragma unroll
+                    for (int i = 0; i < kprime; ++i)
+                        sbuf[i * 64 + m] = (uint32_t)(L[i] == KEY_EMPTY ? KEY_EMPTY : L[i] + 32);
+                }
+                __syncthreads();
+                qact = qmine;
+                wb = h == 0 ? lane4 + 256 * kprime : lane4;
+            }
+            while (__any(wb > lane4)) compact_half();
+        }
         if (lane == 0) next = claim();
 
         // ---- finalize: certification and the list of candidates the FP64 re-rank must see
@@ -703,6 +816,7 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
             }
         }
         __syncthreads();
+        bool to_exact = false;
         if (qmine) {
             const int k = p.k;
             const int nl = ncand < kprime ? ncand : kprime;
@@ -739,10 +853,10 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                 a.cand_eps[q_mine] = eps;
             } else {
                 a.cand_n[q_mine] = -1;
-                const int f = atomicAdd(a.n_fallback, 1);
-                a.fallback[f] = q_mine;
+                to_exact = true;
             }
         }
+        wave_append(a.fallback, a.n_fallback, to_exact, q_mine);
         };
         // long candidate streams keep a deeper list: their keys carry more position bits (coarser
         // distances) and their neighbour distances are closer together
@@ -767,7 +881,9 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
 // precede it).  A window overflow (more than W mutually ambiguous candidates) sends the query
 // on: the first pass keeps a short window (its predicated insertion and shifts are most of the
 // VALU work) and hands overflows to a second pass over that list with the long window; what
-// overflows there goes to the exact kernel.
+// overflows there goes to the exact kernel.  (The pass is bound by the gathers' path through
+// the vector memory unit, not by the FP64 arithmetic: a first pass in FP32 with a certified
+// margin halves the VALU instructions and leaves the time unchanged.)
 constexpr int WIN1 = 8;    // first pass: every certified query
 constexpr int WIN2 = 16;   // second pass: the first pass's overflows
 
@@ -892,10 +1008,8 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
             else knn_col[(int64_t)j * n_q + q] = last_c;
         }
         dk2[q] = last_t;
-    } else if (qv && rn >= 0) {
-        const int f = atomicAdd(n_spill, 1);
-        spill[f] = (int)q;
     }
+    wave_append(spill, n_spill, qv && rn >= 0 && !done, (int)q);
     // the buffered lists leave slot by slot: 64 queries per store
     if (k <= 32) {
         for (int j = 0; j < k; ++j)
@@ -1017,9 +1131,74 @@ __global__ void k_mfma_probe(float* out) {
     }
 }
 
+// Stable split of every XCD segment [seg_beg[x], seg_beg[x + 1]) of items into its items of more
+// than 32 queries, then those of at most 32: out[seg_beg[x] .. seg_half[x]) and
+// out[seg_half[x] .. seg_beg[x + 1]).  One workgroup per segment.
+__global__ __launch_bounds__(1024) void k_split_items(const ScreenItem* __restrict__ in, ScreenItem* __restrict__ out,
+                                                     const int* __restrict__ seg_beg, int* __restrict__ seg_half) {
+    __shared__ int s_wave[16];
+    __shared__ int s_tot;
+    const int x = blockIdx.x, t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int b = seg_beg[x], e = seg_beg[x + 1];
+    // pass 1: items of the full kind
+    int nf = 0;
+    for (int i = b + t; i < e; i += 1024) nf += in[i].nq > 32;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) nf += __shfl_xor(nf, off, 64);
+    if (l == 0) s_wave[w] = nf;
+    __syncthreads();
+    if (t == 0) {
+        int tot = 0;
+        for (int i = 0; i < 16; ++i) tot += s_wave[i];
+        s_tot = tot;
+        seg_half[x] = b + tot;
+    }
+    __syncthreads();
+    const int half0 = b + s_tot;
+    // pass 2: ranks within each kind, chunk by chunk
+    int done_f = 0, done_h = 0;
+    for (int c0 = b; c0 < e; c0 += 1024) {
+        const int i = c0 + t;
+        const bool v = i < e;
+        ScreenItem it{};
+        if (v) it = in[i];
+        const bool full = v && it.nq > 32;
+        const uint64_t bf = __ballot(full), bv = __ballot(v);
+        const int rf = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bf >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bf, 0u));
+        const int rv = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bv >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bv, 0u));
+        __syncthreads();
+        if (l == 0) s_wave[w] = __popcll(bf) | (__popcll(bv) << 16);
+        __syncthreads();
+        int pf = 0, pv = 0;
+        for (int j = 0; j < w; ++j) {
+            pf += s_wave[j] & 0xffff;
+            pv += s_wave[j] >> 16;
+        }
+        int cf = 0, cv = 0;
+        for (int j = 0; j < 16; ++j) {
+            cf += s_wave[j] & 0xffff;
+            cv += s_wave[j] >> 16;
+        }
+        if (v) {
+            const int kf = pf + rf, kh = (pv + rv) - kf;   // full / half items before this one
+            if (full) out[b + done_f + kf] = it;
+            else out[half0 + done_h + kh] = it;
+        }
+        done_f += cf;
+        done_h += cv - cf;
+    }
+    if (x == 7 && t == 0) seg_half[8] = e;
+}
+
 template <int PREC, int KB>
-hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* seg_ctr, const int* seg_beg) {
-#define IRC_SK(KPB) k_screen_knn<PREC, KB, KPB><<<grid, 64, 0, st>>>(a, seg_ctr, seg_beg)
+hipError_t launch_kb(hipStream_t st, const ScreenArgs& a, int grid, int* ctr, const int* seg_beg) {
+    // ctr: [0, 8) counters of the full kernel, [8, 16) of the half kernel, [16, 25) split offsets
+    int* seg_half = ctr + 16;
+    k_zero_ints<<<1, 64, 0, st>>>(ctr, 16);
+    k_split_items<<<8, 1024, 0, st>>>(a.items, a.items_kind, seg_beg, seg_half);
+#define IRC_SK(KPB)                                                                                         \
+    k_screen_knn<PREC, KB, KPB, false><<<grid, 64, 0, st>>>(a, ctr, seg_beg, seg_half);                     \
+    k_screen_knn<PREC, KB, KPB, true><<<grid, 64, 0, st>>>(a, ctr + 8, seg_half, seg_beg + 1)
     switch (a.kprime) {
         case 32: IRC_SK(32); break;
         case 40: IRC_SK(40); break;
@@ -1099,7 +1278,6 @@ hipError_t launch_site_scale(hipStream_t st, int n_seg, const int64_t* seg_ptr, 
 hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KB, const ScreenArgs& a, int grid, int* seg_ctr,
                                const int* seg_beg) {
     if (a.n_items <= 0) return hipSuccess;
-    k_zero_ints<<<1, 64, 0, st>>>(seg_ctr, 8);
     if (prec == SCREEN_FP8) return launch_kb<SCREEN_FP8, 4>(st, a, grid, seg_ctr, seg_beg);
     if (prec == SCREEN_BF16) {
         switch (KB) {
@@ -1124,7 +1302,7 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
     if (a.n_q <= 0) return hipSuccess;
     k_zero_ints<<<1, 64, 0, st>>>(n_spill, 1);
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
-    const unsigned grid2 = grid < 1024 ? grid : 1024;
+    const unsigned grid2 = grid < 1024 ? grid : 1024;   // the second pass takes few queries
     const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
