@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--host", action="store_true",
                     help="also time the host-buffer form of the headline call (features copied in and results "
                          "copied out over PCIe inside the timed call)")
+    ap.add_argument("--no-lanes", action="store_true",
+                    help="skip the launch-pattern lane on lognormal-10M (persistent queue vs per-site launches vs "
+                         "graph replay of the autotuned setting)")
+    ap.add_argument("--lane-steps", type=int, default=3)
     ap.add_argument("--copy-calib", type=float, default=0.0,
                     help="only run a plain device copy of this many GiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -114,6 +118,7 @@ def main():
             "ms_device_total": st_acc["ms_total"] / a.steps,
             "knn_kernel_ms": ms_kernel, "rerank_ms": st_acc["ms_rerank"] / a.steps,
             "knn_tflops": flops / (ms_kernel * 1e-3) / 1e12,
+            "roofline_frac": flops / (ms_kernel * 1e-3) / PEAK[form],
             "screen_fallback": int(s["n_screen_fallback"]), "rerank_overflow": int(s["n_rerank_overflow"]),
             "delta_full_scan": int(s["n_delta_full"]),
             "precision_used": int(s["precision_used"]),
@@ -178,7 +183,8 @@ def main():
                      "frac": ach / peak, "traffic": traffic,
                      "kernel": "kNN pass (screen kernel when screened)",
                      "flops_per_launch": flops,
-                     "traffic_source": TRAFFIC_SRC if traffic is not None else None},
+                     "traffic_source": TRAFFIC_SRC if traffic is not None else None,
+                     "frac_by_form": {f: forms[f]["roofline_frac"] for f in forms}},
         "forms": forms,
     }
     if tuned:
@@ -189,6 +195,9 @@ def main():
         line["per_site_launches"] = per_site(ds, lay, a, ctx, PREC[head], results[head])
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ds, a)
+    if not a.no_lanes and a.workload == "drift-10M":
+        del results, out, dev, lay, ds
+        line["launch_lane"] = launch_lane(a, ctx, head)
     print(json.dumps(line), flush=True)
 
 
@@ -197,17 +206,65 @@ TRAFFIC_SRC = ("profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE and WRITE
 
 
 def screen_traffic(workload, form):
-    """HBM bytes per launch of the form's screen kernel from the committed counter summary, or None."""
+    """HBM bytes of one screen launch (its kernels: the item split, the full-item and the half-item
+    screen kernels) from the committed counter summary, or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
-    if form not in ("bf16", "fp32") or not os.path.exists(path):
+    if form not in ("bf16", "fp32", "fp8") or not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    prefix = "k_screen_knn<1," if form == "bf16" else "k_screen_knn<0,"
+    prefix = "k_screen_knn<" + {"fp32": "0", "bf16": "1", "fp8": "2"}[form] + ","
+    tot, hit = 0.0, False
     for name, d in t.get("forms", {}).get(form, {}).items():
-        if name.startswith(prefix):
-            return d["hbm_bytes"]
-    return None
+        if name.startswith(prefix) or name.startswith("k_split_items"):
+            tot += d["hbm_bytes"]
+            hit = hit or name.startswith(prefix)
+    return tot if hit else None
+
+
+def launch_lane(a, ctx, form):
+    """lognormal-10M (lognormal site sizes, Dirichlet chunk shares): the batch call with the
+    persistent per-XCD item queue, with one screen launch per site (the per-site launch pattern of
+    the reference's site loop, matlab/irc2.m:3537-3561, 3764-3788), and as a graph replay of the
+    setting the library's autotuner picks (irc_autotune).  Each is timed over `lane_steps` calls
+    after one warm-up; all three must give identical outputs."""
+    import torch
+    from ironclust_amd import irc2, libirc, synth
+    ds = synth.make_workload("lognormal-10M", device="cuda")
+    lay = irc2.site_layout(ds)
+    N, k = ds["n_spikes"], ds["k"]
+    dev = dict(F=lay["F"], col_spike=lay["col_spike"], site_ptr=lay["site_ptr"],
+               site_n_primary=lay["site_n_primary"], lim_drift=lay["lim_drift"], link=lay["link"],
+               n_spikes=N, k=k)
+    out = dict(rho=torch.empty(N, dtype=torch.float32, device="cuda"),
+               delta=torch.empty(N, dtype=torch.float32, device="cuda"),
+               nneigh=torch.empty(N, dtype=torch.int64, device="cuda"),
+               knn=torch.empty((N, k), dtype=torch.int64, device="cuda"))
+    prec = PREC[form]
+
+    def timed(opts):
+        ctx.rho_delta_batch_device(dev, out, opts)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.lane_steps):
+            ctx.rho_delta_batch_device(dev, out, opts)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / a.lane_steps
+        return {"ms_per_step": dt * 1e3, "spikes_per_s": N / dt, "ms_screen": ctx.stats()["ms_screen"]}
+
+    res = {"workload": "lognormal-10M", "form": form, "n_spikes": N}
+    res["persistent_queue"] = timed(libirc.make_options(precision=prec))
+    ref = {kk: v.clone() for kk, v in out.items()}
+    res["site_launches"] = timed(libirc.make_options(precision=prec, flags=libirc.FLAG_SITE_LAUNCHES))
+    same = all(torch.equal(out[kk], ref[kk]) for kk in out)
+    choice, rep = ctx.autotune_device(dev, out, libirc.make_options(precision=prec))
+    res["autotune"] = {"choice": choice, "ms_base": rep["ms_base"], "ms_best": rep["ms_best"],
+                       "trials": len(rep["trials"])}
+    res["graph_autotuned"] = timed(libirc.make_options(precision=prec, flags=libirc.FLAG_GRAPH, **choice))
+    same = same and all(torch.equal(out[kk], ref[kk]) for kk in out)
+    res["identical"] = same
+    print(f"# launch lane: {res}", file=sys.stderr, flush=True)
+    return res
 
 
 def per_site(ds, lay, a, ctx, prec, ref):

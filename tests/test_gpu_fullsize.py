@@ -7,7 +7,8 @@ the denser neighbour really denser, rho = 1/d_k.
   drift-10M      384 sites, 10M spikes, D=42, 200 chunks, 15 links, k=30 (16 sites, both forms)
   lognormal-10M  lognormal site sizes (500..600k spikes) and Dirichlet chunk shares (3 sites, one
                  from the top decile of site sizes; both forms)
-  multiday-100M  100M spikes, 2000 chunks (3 sites)
+  multiday-100M  100M spikes, 2000 chunks (the smallest and largest site; BF16, and FP8 whose
+                 certification rate is printed)
 """
 import numpy as np
 import pytest
@@ -18,7 +19,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-FORMS = {"bf16": libirc.PREC_BF16_SCREEN, "fp32": libirc.PREC_FP32}
+FORMS = {"bf16": libirc.PREC_BF16_SCREEN, "fp32": libirc.PREC_FP32, "fp8": libirc.PREC_FP8_SCREEN}
 
 
 def _run(ds, prec, k):
@@ -122,11 +123,16 @@ def test_lognormal_10m_sites_and_invariants(form):
     _check_invariants(ds, out)
 
 
-def test_multiday_100m_sites_and_invariants():
+@pytest.mark.parametrize("form", ["bf16", "fp8"])
+def test_multiday_100m_sites_and_invariants(form):
     ds = synth.make_workload("multiday-100M", device="cuda")
-    out, st = _run(ds, libirc.PREC_BF16_SCREEN, ds["k"])
+    out, st = _run(ds, FORMS[form], ds["k"])
     assert st["n_queries"] == 100_000_000
-    assert st["n_screen_fallback"] < 1e-4 * ds["n_spikes"], st
+    assert st["precision_used"] == FORMS[form]
+    # FP8's bound is looser: the queries it cannot certify take the exact kernel (results stay exact)
+    print(f"multiday-100M {form}: {st['n_screen_fallback']} of {st['n_queries']} queries to the exact kernel")
+    if form == "bf16":
+        assert st["n_screen_fallback"] < 1e-4 * ds["n_spikes"], st
     size = _site_sizes(ds)
     order = np.argsort(size, kind="stable")
     _check_sites(ds, out, [int(order[0]) + 1, int(order[-1]) + 1])
