@@ -396,9 +396,17 @@ struct DevWork {
 // from `ranges` (host planning, per-site entry points) or is built on the device (`dw`, batch).
 // feat_out: when set, the [n_cols][DP] FP32 rows (p.feat) are written here from F (stride D) --
 // by the screen's packing kernel in the same pass over F, or by the FP32 packer on the exact path
+// fin: batch outputs (rho, d_k, int64 ids) the screened path finishes in the re-rank and, for the
+// exact kernel's queries, in a list pass; *fin_done tells the caller whether that happened
+struct FinishOut {
+    float* rho;
+    float* dk;
+    int64_t* knn;
+};
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
                    int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
-                   double* dk2, float* feat_out = nullptr) {
+                   double* dk2, float* feat_out = nullptr, const FinishOut* fin = nullptr, bool* fin_done = nullptr) {
+    if (fin_done) *fin_done = false;
     hipStream_t st = ctx->stream;
     int64_t pairs = 0;
     if (ranges)
@@ -554,6 +562,9 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.cand_eps = ptr<double>(ctx->bb->b_cand_eps);
     a.fallback = fb;
     a.n_fallback = nfb;
+    a.fin_rho = fin ? fin->rho : nullptr;
+    a.fin_dk = fin ? fin->dk : nullptr;
+    a.fin_knn = fin ? fin->knn : nullptr;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
     if (dw && (ctx->flags & IRC_FLAG_SITE_LAUNCHES)) {
         // one launch per site over that site's items (blocks are site-major, so a site's items are
@@ -593,6 +604,10 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[6], st));
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));
+    if (irc::screen_rerank_finishes(a)) {
+        HIPCHK(irc::launch_knn_finish_list(st, p, fb, nfb, nq, knn_col, dk2, a.fin_rho, a.fin_dk, a.fin_knn));
+        if (fin_done) *fin_done = true;
+    }
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 1));
     if (!ranges) HIPCHK(irc::launch_stat_copy(st, nullptr, pb.n_pairs, dstat + 2));
     return IRC_OK;
@@ -1052,10 +1067,14 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         auto device_phase = [&]() -> irc_status {
             HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
+            const FinishOut fo{o_rho, o_dk, o_knn};
+            bool fin_done = false;
             CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols,
-                        nullptr, &dw, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), ptr<float>(ctx->bb->b_feat)));
-            HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), o_rho,
-                                          o_dk, nullptr, o_knn));
+                        nullptr, &dw, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), ptr<float>(ctx->bb->b_feat),
+                        &fo, &fin_done));
+            if (!fin_done)
+                HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2),
+                                              o_rho, o_dk, nullptr, o_knn));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
             HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->bb->b_rhocol)));
             HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->bb->b_knncol), ptr<float>(ctx->bb->b_rhocol),

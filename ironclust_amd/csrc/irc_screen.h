@@ -61,6 +61,11 @@ struct ScreenArgs {
     double* cand_eps;        // per query: the screen's error bound
     int* fallback;           // queries to redo with the exact kernel
     int* n_fallback;
+    // batch form: the re-rank also finishes the queries it completes (rho = float(1/sqrt(d2_k)),
+    // d_k, and the int64 spike ids of the list in slot order, row q_out * k); null: not fused
+    float* fin_rho;
+    float* fin_dk;
+    int64_t* fin_knn;
 };
 
 int screen_kp(int D);                                  // padded operand width of the FP32/BF16 forms
@@ -88,6 +93,8 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KB, const ScreenArg
 // prefix does not fit that either join the fallback list.
 hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2, int* spill,
                                 int* n_spill);
+// the re-rank finishes its queries itself (fin_* set) only for lists of at most 32
+inline bool screen_rerank_finishes(const ScreenArgs& a) { return a.p.k <= 32 && (a.fin_rho || a.fin_knn); }
 hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 // largest |x|^2 of every (block, part) column run
 hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,

@@ -888,13 +888,19 @@ constexpr int WIN1 = 8;    // first pass: every certified query
 constexpr int WIN2 = 16;   // second pass: the first pass's overflows
 
 // one group of up to 64 queries (lane l = query q, valid when qv); overflows are appended to spill
+struct FinOut {
+    float* rho;
+    float* dk;
+    int64_t* knn;
+};
+
 template <int NV, int W>
 __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int64_t q, bool qv,
                                              const int* __restrict__ cand_n, const int* __restrict__ cand_col,
                                              const float* __restrict__ cand_d, const double* __restrict__ cand_eps,
                                              int* __restrict__ knn_col, double* __restrict__ dk2,
                                              int* __restrict__ spill, int* __restrict__ n_spill, float4* stage,
-                                             int* skey, float* sdn, int* obuf) {
+                                             int* skey, float* sdn, int* obuf, const FinOut& fin) {
     const int l = threadIdx.x;
     const int rn = qv ? cand_n[q] : -1;
     const int r = rn < 0 ? 0 : (rn & 0xffff);
@@ -1014,6 +1020,27 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
     if (k <= 32) {
         for (int j = 0; j < k; ++j)
             if (done) knn_col[(int64_t)j * n_q + q] = obuf[j * 64 + l];
+        // batch form: rho, d_k and the reported ids of the finished queries (the finish of
+        // rho_paged_site_, matlab/irc2.m:3669-3676): the list's columns become spike ids in place,
+        // then each finished query's row of k ids leaves contiguously (consecutive lanes store
+        // consecutive entries of one row)
+        if (fin.rho || fin.knn) {
+            const int slot = done ? p.q_out[q] : 0;
+            if (done && fin.rho) fin.rho[slot] = (float)(1.0 / sqrt(last_t));
+            if (done && fin.dk) fin.dk[slot] = (float)sqrt(last_t);
+            if (fin.knn) {
+                if (done)
+                    for (int j = 0; j < k; ++j) obuf[j * 64 + l] = p.col_ret[obuf[j * 64 + l]];
+                __syncthreads();
+                const uint64_t dm = __ballot(done);
+                for (int e = l; e < k * 64; e += 64) {
+                    const int c = e / k, j = e - c * k;
+                    const int sc = __shfl(slot, c, 64);
+                    if ((dm >> c) & 1ull) fin.knn[(int64_t)sc * k + j] = obuf[j * 64 + c];
+                }
+            }
+        }
+        __syncthreads();   // obuf is reused by the next group
     }
 }
 
@@ -1022,7 +1049,7 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
                                               const int* __restrict__ cand_col, const float* __restrict__ cand_d,
                                               const double* __restrict__ cand_eps, int* __restrict__ knn_col,
                                               double* __restrict__ dk2, int* __restrict__ spill,
-                                              int* __restrict__ n_spill) {
+                                              int* __restrict__ n_spill, FinOut fin) {
     __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
     __shared__ int skey[64];            // their spike ids
     __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
@@ -1034,7 +1061,7 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
                                                : lo8 * per + (x - lo8) * (per - 1) + (blockIdx.x >> 3);
     const int64_t q = (int64_t)grp * 64 + threadIdx.x;
     rerank_group<NV, WIN1>(p, n_q, q, q < n_q, cand_n, cand_col, cand_d, cand_eps, knn_col, dk2, spill, n_spill,
-                           stage, skey, sdn, obuf);
+                           stage, skey, sdn, obuf, fin);
 }
 
 // second pass: the queries of list[0 .. *n_list), 64 per group, groups strided over the grid
@@ -1044,7 +1071,7 @@ __global__ __launch_bounds__(64) void k_rerank_list(DevPlan p, int64_t n_q, cons
                                                    const int* __restrict__ cand_col, const float* __restrict__ cand_d,
                                                    const double* __restrict__ cand_eps, int* __restrict__ knn_col,
                                                    double* __restrict__ dk2, int* __restrict__ fallback,
-                                                   int* __restrict__ n_fallback) {
+                                                   int* __restrict__ n_fallback, FinOut fin) {
     __shared__ float4 stage[64 * NV];
     __shared__ int skey[64];
     __shared__ float sdn[64];
@@ -1054,7 +1081,7 @@ __global__ __launch_bounds__(64) void k_rerank_list(DevPlan p, int64_t n_q, cons
         const int64_t i = g * 64 + threadIdx.x;
         const bool qv = i < n;
         rerank_group<NV, WIN2>(p, n_q, qv ? (int64_t)list[i] : 0, qv, cand_n, cand_col, cand_d, cand_eps, knn_col,
-                               dk2, fallback, n_fallback, stage, skey, sdn, obuf);
+                               dk2, fallback, n_fallback, stage, skey, sdn, obuf, fin);
     }
 }
 
@@ -1304,12 +1331,14 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
     const unsigned grid2 = grid < 1024 ? grid : 1024;   // the second pass takes few queries
     const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
+    FinOut fin{nullptr, nullptr, nullptr};
+    if (screen_rerank_finishes(a)) fin = FinOut{a.fin_rho, a.fin_dk, a.fin_knn};
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
         k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, \
-                                          dk2, spill, n_spill);                                            \
+                                          dk2, spill, n_spill, fin);                                       \
         k_rerank_list<NV><<<grid2, 64, obytes, st>>>(a.p, a.n_q, spill, n_spill, a.cand_n, a.cand_col, a.cand_d, \
-                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback);       \
+                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback, fin);  \
         break;
     switch (a.p.DP / 4) {
         IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)

@@ -88,3 +88,61 @@ def test_isolation_mode_ties_and_empty():
     assert nn[0] == 2 and s[0] == 1 / 2
     # clusters 2..4 only see themselves: score 1, no neighbour (NaN)
     assert np.all(s[1:] == 1) and np.all(np.isnan(nn[1:]))
+
+
+def _chain_jump_interleaved(lab0, nneigh, rng, p_stale):
+    """The device's pointer jumping (k_chain_init, then ceil(log2 n) + 2 rounds of k_chain_jump,
+    ironclust_amd/csrc/irc_assign.hip) run on the host with the threads of each round in a random
+    order and every read of another spike's lab / nxt entry seeing, at random, the value at the
+    start of the round or the one already written in this round -- the interleavings a GPU round
+    may produce, since lab and nxt are updated in place."""
+    n = len(lab0)
+    lab = lab0.astype(np.int64).copy()
+    j0 = nneigh.astype(np.int64) - 1
+    nxt = np.where((lab > 0) | (j0 < 0) | (j0 >= n), np.arange(n), j0)
+    rounds = 1
+    while (1 << (rounds - 1)) < n:
+        rounds += 1
+    for _ in range(rounds + 1):
+        lab_old, nxt_old = lab.copy(), nxt.copy()
+        for i in rng.permutation(n):
+            if lab[i] > 0:
+                continue
+            j = nxt[i]
+            lj = lab_old[j] if rng.random() < p_stale else lab[j]
+            if lj > 0:
+                lab[i] = lj
+            else:
+                nxt[i] = nxt_old[j] if rng.random() < p_stale else nxt[j]
+    return lab
+
+
+def test_chain_jump_result_does_not_depend_on_interleaving():
+    """k_chain_jump reads neighbours' entries while other threads write them; whatever mix of old and
+    new values a round sees, every spike must end with the label of the first peak on its chain
+    within the launched number of rounds.  Long chains (a line with monotone density) and a random
+    density-peak forest from the oracle are both checked under many interleavings."""
+    rng = np.random.default_rng(5)
+    # a single path n -> n-1 -> ... -> 1 (peak at spike 1) plus a second peak mid-way
+    n = 300
+    nneigh = np.concatenate([[1], np.arange(1, n)]).astype(np.int64)
+    lab0 = np.zeros(n, np.int32)
+    lab0[0] = 1
+    lab0[150] = 2
+    want = _chain_walk_labels(lab0, nneigh)
+    for p in (0.0, 0.5, 1.0):
+        for _ in range(5):
+            assert np.array_equal(_chain_jump_interleaved(lab0, nneigh, rng, p), want)
+    # the nearest-denser forest of a small recording, peaks labelled by delta
+    ds = synth.make_dataset(3000, 2, 5, 3, 3, k=8, drift_step=0.3, n_links=2, band=1, seed=77)
+    r = O.sort_batch(ds)
+    nn = r["nneigh"].astype(np.int64)
+    order = np.argsort(-r["delta"])
+    lab0 = np.zeros(len(nn), np.int32)
+    lab0[order[:12]] = np.arange(1, 13)
+    lab0[nn - 1 == np.arange(len(nn))] = 13    # every chain end (no denser spike) is a peak
+    want = _chain_walk_labels(lab0, nn)
+    assert (want > 0).all()
+    for p in (0.0, 0.3, 0.7, 1.0):
+        for _ in range(3):
+            assert np.array_equal(_chain_jump_interleaved(lab0, nn, rng, p), want)
