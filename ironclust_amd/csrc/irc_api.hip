@@ -39,7 +39,7 @@ struct DevBuf {
 struct Bufs {
     DevBuf b_F, b_colspk, b_feat, b_key, b_ret, b_runb, b_runl, b_lptr, b_lidx, b_qcol, b_qblk, b_qout;
     DevBuf b_small, b_flags, b_knncol, b_dk2, b_rhoslot, b_rhocol, b_unres, b_out_delta, b_out_nn,
-        b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand;
+        b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand, b_d2head;
     DevBuf b_scr[6];   // screen: [0] operand rows, [1] FP8 site scales, [2] norms + run maxima, [3] items,
                        // [4] fallback/spill lists, [5] counters
     DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
@@ -57,7 +57,7 @@ struct Bufs {
                                &b_qcol, &b_qblk, &b_qout, &b_small, &b_flags, &b_knncol, &b_dk2, &b_rhoslot,
                                &b_rhocol, &b_unres, &b_out_delta, &b_out_nn, &b_out_knn, &b_out_dk, &b_out_nn32,
                                &b_out_knn32, &b_ncand, &b_cand_n, &b_cand_col, &b_cand_d, &b_cand_eps, &b_plan,
-                               &b_drift, &b_assign, &b_siteseg, &b_tune, &b_dstat};
+                               &b_drift, &b_assign, &b_siteseg, &b_tune, &b_dstat, &b_d2head};
         for (auto& b : b_scr) v.push_back(&b);
         return v;
     }
@@ -402,6 +402,7 @@ struct FinishOut {
     float* rho;
     float* dk;
     int64_t* knn;
+    double* d2head;   // exact d^2 of the first IRC_D2HEAD entries of every list, for the delta pass
 };
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
                    int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
@@ -565,6 +566,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.fin_rho = fin ? fin->rho : nullptr;
     a.fin_dk = fin ? fin->dk : nullptr;
     a.fin_knn = fin ? fin->knn : nullptr;
+    a.fin_d2head = fin ? fin->d2head : nullptr;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
     if (dw && (ctx->flags & IRC_FLAG_SITE_LAUNCHES)) {
         // one launch per site over that site's items (blocks are site-major, so a site's items are
@@ -998,6 +1000,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         CHK(ensure(ctx, ctx->bb->b_dk2, sizeof(double) * (size_t)N));
         CHK(ensure(ctx, ctx->bb->b_rhocol, sizeof(float) * (size_t)n_cols));
         CHK(ensure(ctx, ctx->bb->b_unres, sizeof(int) * ((size_t)N + 1)));
+        CHK(ensure(ctx, ctx->bb->b_d2head, sizeof(double) * IRC_D2HEAD * (size_t)N));
         int64_t* d_site_ptr = ptr<int64_t>(ctx->bb->b_small);
         int64_t* d_site_np = d_site_ptr + (S + 1);
         int64_t* d_prim = d_site_np + S;
@@ -1067,7 +1070,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         auto device_phase = [&]() -> irc_status {
             HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
-            const FinishOut fo{o_rho, o_dk, o_knn};
+            const FinishOut fo{o_rho, o_dk, o_knn, ptr<double>(ctx->bb->b_d2head)};
             bool fin_done = false;
             CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols,
                         nullptr, &dw, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), ptr<float>(ctx->bb->b_feat),
@@ -1078,7 +1081,8 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
             HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->bb->b_rhocol)));
             HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->bb->b_knncol), ptr<float>(ctx->bb->b_rhocol),
-                                              o_delta, o_nn, ptr<int>(ctx->bb->b_unres), d_nunres));
+                                              o_delta, o_nn, ptr<int>(ctx->bb->b_unres), d_nunres,
+                                              fin_done ? ptr<double>(ctx->bb->b_d2head) : nullptr));
             HIPCHK(irc::launch_delta_exact(st, p, ptr<int>(ctx->bb->b_unres), nq, d_nunres, persistent_grid(ctx, 16), 0,
                                            ptr<float>(ctx->bb->b_rhocol), o_delta, o_nn, nullptr));
             HIPCHK(irc::launch_stat_copy(st, d_nunres, nullptr, ptr<int64_t>(ctx->bb->b_dstat) + 4));

@@ -6,6 +6,7 @@
 #define IRC_WAVE 64
 #define IRC_DMAX 64          // largest padded feature dimension handled by the kernels
 #define IRC_KMAX 64          // largest k handled by the kernels
+#define IRC_D2HEAD 4         // list entries whose exact d^2 the re-rank keeps for the delta pass
 
 // The drop-in value reported for a spike with no denser neighbour:
 // single(sqrt(3.402e38)) (matlab/irc2.m:3686, 3723-3725).

@@ -267,11 +267,13 @@ __global__ void k_rho_cols(int64_t n_cols, const int* __restrict__ col_key, cons
 // ---------------------------------------------------------------------------
 // Delta from the kNN list: the first kNN entry (in (d2,id) order) whose rho exceeds
 // the query's is the nearest denser spike, because kNN(a) is a prefix of C(a) in key
-// order; its canonical distance is recomputed here (one row pair per query).  Queries
-// without such an entry are appended to `unres`.
+// order.  Its canonical distance is taken from d2head (the exact d^2 of the first IRC_D2HEAD
+// entries, written by the re-rank; NaN where unknown) or recomputed here (one row pair per query).
+// Queries without such an entry are appended to `unres`.
 template <int NV>
 __global__ __launch_bounds__(64) void k_delta_from_knn(DevPlan p, int nq, const int* __restrict__ knn_col,
                                                        const float* __restrict__ rho_col,
+                                                       const double* __restrict__ d2head,
                                                        float* __restrict__ delta_slot,
                                                        int64_t* __restrict__ nneigh_slot, int* __restrict__ unres,
                                                        int* __restrict__ n_unres) {
@@ -284,57 +286,83 @@ __global__ __launch_bounds__(64) void k_delta_from_knn(DevPlan p, int nq, const 
     const bool qv = q < nq;
     const int qc = p.q_col[qv ? q : nq - 1];
     const float ra = rho_col[qc];
-    int found = -1;
+    // the list is scanned eight entries at a time: their columns, then their rho, each as eight
+    // independent loads (two memory round trips per step instead of two per entry)
+    int found = -1, jf = -1;
     if (qv) {
-        for (int j = 0; j < p.k; ++j) {
-            const int col = knn_col[(int64_t)j * p.nq + q];
-            if (rho_col[col] > ra) {
-                found = col;
-                break;
+        for (int j0 = 0; j0 < p.k && found < 0; j0 += 8) {
+            int cols[8];
+            float rh[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cols[u] = j0 + u < p.k ? knn_col[(int64_t)(j0 + u) * p.nq + q] : qc;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) rh[u] = rho_col[cols[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (found < 0 && j0 + u < p.k && rh[u] > ra) {
+                    found = cols[u];
+                    jf = j0 + u;
+                }
             }
         }
     }
+    double t = __builtin_nan("");
+    if (found >= 0 && d2head && jf < IRC_D2HEAD) t = d2head[(int64_t)jf * p.nq + q];
+    const bool need = found >= 0 && __builtin_isnan(t);
+    const uint64_t nmask = __ballot(need);
+    // the rows of the lanes in nmask: load instruction i carries pieces of the rows of lanes
+    // i*64/NV .. (i*64+63)/NV and is skipped when none of them is needed; the pieces of unneeded rows
+    // it does carry come from row 0
     auto gather = [&](int col) {
         int crow[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) crow[i] = __shfl(col, (i * 64 + l) / NV, 64);
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
+            constexpr uint64_t one = 1;
+            const int o_lo = i * 64 / NV, o_hi = (i * 64 + 63) / NV;
+            const uint64_t span = (o_hi - o_lo == 63 ? ~(uint64_t)0 : ((one << (o_hi - o_lo + 1)) - 1)) << o_lo;
+            if ((nmask & span) == 0) continue;
             const int piece = i * 64 + l;
-            const int part = piece - (piece / NV) * NV;
-            const float* src = p.feat + (int64_t)crow[i] * p.DP + 4 * part;
+            const int owner = piece / NV;
+            const int part = piece - owner * NV;
+            const int row = ((nmask >> owner) & 1ull) ? crow[i] : 0;
+            const float* src = p.feat + (int64_t)row * p.DP + 4 * part;
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                              (__attribute__((address_space(3))) void*)(stage + i * 64), 16, 0, 0);
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the rows have landed in LDS
     };
-    gather(qc);
-    float4 a[NV];
+    if (nmask) {
+        gather(qc);
+        float4 a[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) a[i] = stage[l * NV + i];
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // read back before the next gather lands
-    if (__any(found >= 0)) {
+        for (int i = 0; i < NV; ++i) a[i] = stage[l * NV + i];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // read back before the next gather lands
         gather(found >= 0 ? found : qc);
-        if (found >= 0) {
-            double t = 0.0;
+        if (need) {
+            double u = 0.0;
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 const float4 x = a[i], y = stage[l * NV + i];
                 double d;
-                d = (double)x.x - (double)y.x; t = fma(d, d, t);
-                d = (double)x.y - (double)y.y; t = fma(d, d, t);
-                d = (double)x.z - (double)y.z; t = fma(d, d, t);
-                d = (double)x.w - (double)y.w; t = fma(d, d, t);
+                d = (double)x.x - (double)y.x; u = fma(d, d, u);
+                d = (double)x.y - (double)y.y; u = fma(d, d, u);
+                d = (double)x.z - (double)y.z; u = fma(d, d, u);
+                d = (double)x.w - (double)y.w; u = fma(d, d, u);
             }
-            const int slot = p.q_out[q];
-            const float dl = (float)(sqrt(t) * (double)ra);
-            if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
-                delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
-                nneigh_slot[slot] = p.col_ret[qc];
-            } else {
-                delta_slot[slot] = dl;
-                nneigh_slot[slot] = p.col_ret[found];
-            }
+            t = u;
+        }
+    }
+    if (found >= 0) {
+        const int slot = p.q_out[q];
+        const float dl = (float)(sqrt(t) * (double)ra);
+        if (__builtin_isnan(dl) || __builtin_isinf(dl)) {
+            delta_slot[slot] = (float)sqrt(IRC_SINGLE_INF_D);
+            nneigh_slot[slot] = p.col_ret[qc];
+        } else {
+            delta_slot[slot] = dl;
+            nneigh_slot[slot] = p.col_ret[found];
         }
     }
     if (qv && found < 0) {
@@ -638,13 +666,13 @@ hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, c
 
 hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col,
                                  const float* rho_col, float* delta_slot, int64_t* nneigh_slot, int* unres,
-                                 int* n_unres) {
+                                 int* n_unres, const double* d2head) {
     if (nq <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((nq + 63) / 64);
 #define IRC_DK(NV)                                                                                       \
     case NV:                                                                                             \
-        k_delta_from_knn<NV><<<grid, 64, 0, st>>>(p, nq, knn_col, rho_col, delta_slot, nneigh_slot, unres, \
-                                                  n_unres);                                              \
+        k_delta_from_knn<NV><<<grid, 64, 0, st>>>(p, nq, knn_col, rho_col, d2head, delta_slot, nneigh_slot, \
+                                                  unres, n_unres);                                       \
         break;
     switch (p.DP / 4) {
         IRC_DK(1) IRC_DK(2) IRC_DK(3) IRC_DK(4) IRC_DK(5) IRC_DK(6) IRC_DK(7) IRC_DK(8)

@@ -27,9 +27,10 @@ hipError_t launch_knn_finish_list(hipStream_t st, const DevPlan& p, const int* l
 hipError_t launch_knn_dists(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, float* out);
 hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, const float* rho_slot,
                            float* rho_col);
+// d2head (optional): exact d^2 of the first IRC_D2HEAD entries of every list ([slot][nq], NaN unknown)
 hipError_t launch_delta_from_knn(hipStream_t st, const DevPlan& p, int nq, const int* knn_col,
                                  const float* rho_col, float* delta_slot,
-                                 int64_t* nneigh_slot, int* unres, int* n_unres);
+                                 int64_t* nneigh_slot, int* unres, int* n_unres, const double* d2head = nullptr);
 hipError_t launch_delta_exact(hipStream_t st, const DevPlan& p, const int* qlist, int nq, const int* nq_dev,
                               int grid, int mode, const float* rho_col, float* delta_slot, int64_t* nneigh64,
                               int* nneigh32);

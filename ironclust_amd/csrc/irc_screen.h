@@ -66,6 +66,9 @@ struct ScreenArgs {
     float* fin_rho;
     float* fin_dk;
     int64_t* fin_knn;
+    // and the exact d^2 of the first IRC_D2HEAD list entries ([slot][n_q]; NaN for queries it does
+    // not finish), which the delta pass uses instead of recomputing them
+    double* fin_d2head;
 };
 
 int screen_kp(int D);                                  // padded operand width of the FP32/BF16 forms
@@ -94,7 +97,9 @@ hipError_t launch_screen_knn_q(hipStream_t st, int prec, int KB, const ScreenArg
 hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_col, double* dk2, int* spill,
                                 int* n_spill);
 // the re-rank finishes its queries itself (fin_* set) only for lists of at most 32
-inline bool screen_rerank_finishes(const ScreenArgs& a) { return a.p.k <= 32 && (a.fin_rho || a.fin_knn); }
+inline bool screen_rerank_finishes(const ScreenArgs& a) {
+    return a.p.k <= 32 && (a.fin_rho || a.fin_knn || a.fin_d2head);
+}
 hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 // largest |x|^2 of every (block, part) column run
 hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,

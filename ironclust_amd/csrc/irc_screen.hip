@@ -94,12 +94,13 @@ __device__ __forceinline__ float cform(const float* x, int d, int D, float inv_s
     return d < D ? x[d] * inv_s : (d == D ? w : (d == D + 1 ? nrm_sw : 0.0f));
 }
 
-// A workgroup packs 32 columns: their rows are staged in LDS with coalesced loads, one thread per
-// column forms |x|^2 (and, for FP8, looks up its site's scale), then one thread per (column, piece)
-// writes the 16-byte pieces of the candidate-form rows in the lane-half layout of irc_screen.h:
-// consecutive threads take consecutive columns of one piece, so 32 threads store 512 contiguous
-// bytes.  The query-form rows are not stored: the screen derives them per item.
-constexpr int PACK_COLS = 32;
+// A workgroup packs 64 columns: the contiguous span of their rows is staged in LDS as is (16-byte
+// loads when it is aligned), one thread per column forms |x|^2 (and, for FP8, looks up its site's
+// scale), then one thread per (column, piece) writes the 16-byte pieces of the candidate-form rows in
+// the lane-half layout of irc_screen.h: consecutive threads take consecutive columns of one piece, so
+// 32 threads store 512 contiguous bytes.  The query-form rows are not stored: the screen derives them
+// per item.  FS <= IRC_DMAX (the batch passes D, the per-site forms DP), so the span fits.
+constexpr int PACK_COLS = 64;
 
 template <int PREC>
 __global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int FS, int np, const float* __restrict__ F,
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int 
                                                      float* __restrict__ feat, int DP,
                                                      const int64_t* __restrict__ seg_ptr, int n_seg,
                                                      const float* __restrict__ seg_scale) {
-    __shared__ float srow[PACK_COLS * IRC_DMAX];
+    __shared__ float srow[PACK_COLS * IRC_DMAX];   // column c's values at c * FS
     __shared__ float snorm[PACK_COLS];
     __shared__ float sscale[PACK_COLS];
     const int64_t col0 = (int64_t)blockIdx.x * PACK_COLS;
@@ -115,13 +116,18 @@ __global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int 
     // rows col0 .. col0+ncol-1 of F: (ncol-1)*FS + D contiguous floats
     const int span = (ncol - 1) * FS + D;
     const float* src = F + col0 * FS;
-    for (int e = threadIdx.x; e < span; e += blockDim.x) {
-        const int c = e / FS, d = e - c * FS;
-        if (d < D) srow[c * IRC_DMAX + d] = src[e];
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const int n4 = span >> 2;
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(srow);
+        for (int e = threadIdx.x; e < n4; e += blockDim.x) d4[e] = s4[e];
+        for (int e = 4 * n4 + threadIdx.x; e < span; e += blockDim.x) srow[e] = src[e];
+    } else {
+        for (int e = threadIdx.x; e < span; e += blockDim.x) srow[e] = src[e];
     }
     __syncthreads();
     if ((int)threadIdx.x < ncol) {
-        const float* x = srow + threadIdx.x * IRC_DMAX;
+        const float* x = srow + threadIdx.x * FS;
         float n = 0.0f;
         for (int d = 0; d < D; ++d) n = fmaf(x[d], x[d], n);
         snorm[threadIdx.x] = n;
@@ -146,18 +152,18 @@ __global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int 
         float4* f4 = reinterpret_cast<float4*>(feat + col0 * DP);
         for (int e = threadIdx.x; e < ncol * n4; e += blockDim.x) {
             const int cc = e / n4, d0 = 4 * (e - cc * n4);
-            const float* x = srow + cc * IRC_DMAX;
+            const float* x = srow + cc * FS;
             f4[e] = make_float4(d0 < D ? x[d0] : 0.0f, d0 + 1 < D ? x[d0 + 1] : 0.0f, d0 + 2 < D ? x[d0 + 2] : 0.0f,
                                 d0 + 3 < D ? x[d0 + 3] : 0.0f);
         }
     }
     const int nph = np >> 1;
     for (int e = threadIdx.x; e < PACK_COLS * np; e += blockDim.x) {
-        const int c = e & 31, hj = e >> 5;
+        const int c = e & (PACK_COLS - 1), hj = e / PACK_COLS;
         if (c >= ncol) continue;
         const int h = hj / nph, j = hj - h * nph;
         const int64_t col = col0 + c;
-        const float* x = srow + c * IRC_DMAX;
+        const float* x = srow + c * FS;
         uint4 out;
         if (PREC == SCREEN_FP32) {
             const int d0 = 8 * j + 4 * h;
@@ -892,6 +898,8 @@ struct FinOut {
     float* rho;
     float* dk;
     int64_t* knn;
+    double* d2head;   // [IRC_D2HEAD][n_q]
+    int first;        // the first pass visits every query: it marks the ones it does not finish
 };
 
 template <int NV, int W>
@@ -907,7 +915,11 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
     int rmax = r;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) rmax = max(rmax, __shfl_xor(rmax, off, 64));
-    if (rmax == 0) return;
+    if (rmax == 0) {
+        if (fin.d2head && fin.first && qv)
+            for (int j = 0; j < IRC_D2HEAD; ++j) fin.d2head[(int64_t)j * n_q + q] = __builtin_nan("");
+        return;
+    }
     const int qc = p.q_col[qv ? q : 0];
     float4 qr[NV];
     {
@@ -942,6 +954,7 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
     const double eps = qv && rn >= 0 ? cand_eps[q] : 0.0;
     double wt[W];
     int wk[W], wc[W];
+    double dh[IRC_D2HEAD] = {};   // exact d^2 of the first emitted entries
     int wn = 0, out = 0;
     double last_t = 0.0;
     int last_c = -1;
@@ -991,6 +1004,8 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
                 }
                 ++wn;
                 while (wn > 0 && out < kk && wt[0] < lb) {
+#pragma unroll
+                    for (int i = 0; i < IRC_D2HEAD; ++i) dh[i] = out == i ? wt[0] : dh[i];
                     if (k <= 32) obuf[out * 64 + l] = wc[0];
                     else knn_col[(int64_t)out * n_q + q] = wc[0];
                     last_t = wt[0];
@@ -1024,6 +1039,13 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
         // rho_paged_site_, matlab/irc2.m:3669-3676): the list's columns become spike ids in place,
         // then each finished query's row of k ids leaves contiguously (consecutive lanes store
         // consecutive entries of one row)
+        if (fin.d2head) {
+            // padded entries repeat the last one
+            for (int j = 0; j < IRC_D2HEAD; ++j) {
+                if (done) fin.d2head[(int64_t)j * n_q + q] = j < kk ? dh[j] : last_t;
+                else if (fin.first && qv) fin.d2head[(int64_t)j * n_q + q] = __builtin_nan("");
+            }
+        }
         if (fin.rho || fin.knn) {
             const int slot = done ? p.q_out[q] : 0;
             if (done && fin.rho) fin.rho[slot] = (float)(1.0 / sqrt(last_t));
@@ -1331,14 +1353,16 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
     const unsigned grid2 = grid < 1024 ? grid : 1024;   // the second pass takes few queries
     const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
-    FinOut fin{nullptr, nullptr, nullptr};
-    if (screen_rerank_finishes(a)) fin = FinOut{a.fin_rho, a.fin_dk, a.fin_knn};
+    FinOut fin{nullptr, nullptr, nullptr, nullptr, 1};
+    if (screen_rerank_finishes(a)) fin = FinOut{a.fin_rho, a.fin_dk, a.fin_knn, a.fin_d2head, 1};
+    FinOut fin2 = fin;
+    fin2.first = 0;
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
         k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, \
                                           dk2, spill, n_spill, fin);                                       \
         k_rerank_list<NV><<<grid2, 64, obytes, st>>>(a.p, a.n_q, spill, n_spill, a.cand_n, a.cand_col, a.cand_d, \
-                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback, fin);  \
+                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback, fin2); \
         break;
     switch (a.p.DP / 4) {
         IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)
