@@ -403,6 +403,7 @@ struct FinishOut {
     float* dk;
     int64_t* knn;
     double* d2head;   // exact d^2 of the first IRC_D2HEAD entries of every list, for the delta pass
+    float* rho_col;   // rho of the queries' own columns
 };
 irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over, int min_item, const float* F,
                    int FS, int64_t n_cols, const std::vector<QRange>* ranges, const DevWork* dw, int* knn_col,
@@ -567,6 +568,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     a.fin_dk = fin ? fin->dk : nullptr;
     a.fin_knn = fin ? fin->knn : nullptr;
     a.fin_d2head = fin ? fin->d2head : nullptr;
+    a.fin_rho_col = fin ? fin->rho_col : nullptr;
     HIPCHK(irc::launch_fill_int(st, a.cand_n, nq, -1));
     if (dw && (ctx->flags & IRC_FLAG_SITE_LAUNCHES)) {
         // one launch per site over that site's items (blocks are site-major, so a site's items are
@@ -607,7 +609,8 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     // exact kNN for queries outside the screen's limits and for uncertified queries
     HIPCHK(irc::launch_knn_exact_dev(st, p, fb, nfb, nq, persistent_grid(ctx, 8), knn_col, dk2));
     if (irc::screen_rerank_finishes(a)) {
-        HIPCHK(irc::launch_knn_finish_list(st, p, fb, nfb, nq, knn_col, dk2, a.fin_rho, a.fin_dk, a.fin_knn));
+        HIPCHK(irc::launch_knn_finish_list(st, p, fb, nfb, nq, knn_col, dk2, a.fin_rho, a.fin_dk, a.fin_knn,
+                                           a.fin_rho_col));
         if (fin_done) *fin_done = true;
     }
     HIPCHK(irc::launch_stat_copy(st, nfb, nullptr, dstat + 1));
@@ -1070,7 +1073,7 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
         auto device_phase = [&]() -> irc_status {
             HIPCHK(irc::launch_fill_int(st, d_nunres, 1, 0));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[1], st));
-            const FinishOut fo{o_rho, o_dk, o_knn, ptr<double>(ctx->bb->b_d2head)};
+            const FinishOut fo{o_rho, o_dk, o_knn, ptr<double>(ctx->bb->b_d2head), ptr<float>(ctx->bb->b_rhocol)};
             bool fin_done = false;
             CHK(run_knn(ctx, p, nq, resolve_precision(opt), opt ? opt->k_over : 0, min_item(opt), dF, D, n_cols,
                         nullptr, &dw, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2), ptr<float>(ctx->bb->b_feat),
@@ -1079,7 +1082,9 @@ irc_status irc_rho_delta_batch(irc_ctx* ctx, const irc_batch* b, const irc_optio
                 HIPCHK(irc::launch_knn_finish(st, p, nq, 0, ptr<int>(ctx->bb->b_knncol), ptr<double>(ctx->bb->b_dk2),
                                               o_rho, o_dk, nullptr, o_knn));
             if (!ctx->capturing) HIPCHK(hipEventRecord(ctx->ev[2], st));
-            HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->bb->b_rhocol)));
+            // the finish wrote rho for every query's own column: only secondary columns are left
+            if (!fin_done || n_cols > N)
+                HIPCHK(irc::launch_rho_cols(st, n_cols, dspk, o_rho, ptr<float>(ctx->bb->b_rhocol)));
             HIPCHK(irc::launch_delta_from_knn(st, p, nq, ptr<int>(ctx->bb->b_knncol), ptr<float>(ctx->bb->b_rhocol),
                                               o_delta, o_nn, ptr<int>(ctx->bb->b_unres), d_nunres,
                                               fin_done ? ptr<double>(ctx->bb->b_d2head) : nullptr));

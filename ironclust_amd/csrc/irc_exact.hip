@@ -223,7 +223,8 @@ __global__ __launch_bounds__(256) void k_knn_finish(DevPlan p, int nq, int mode,
 // grid-strided -- the queries the exact kernel completed after the re-rank finished the others.
 __global__ void k_knn_finish_list(DevPlan p, const int* __restrict__ list, const int* __restrict__ n_list,
                                   const int* __restrict__ knn_col, const double* __restrict__ dk2,
-                                  float* __restrict__ rho_slot, float* __restrict__ dk_slot, int64_t* __restrict__ knn64) {
+                                  float* __restrict__ rho_slot, float* __restrict__ dk_slot, int64_t* __restrict__ knn64,
+                                  float* __restrict__ rho_col) {
     const int n = *n_list;
     const int k = p.k;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -231,6 +232,7 @@ __global__ void k_knn_finish_list(DevPlan p, const int* __restrict__ list, const
         const int64_t slot = p.q_out[q];
         const double d = dk2[q];
         if (rho_slot) rho_slot[slot] = (float)(1.0 / sqrt(d));
+        if (rho_col) rho_col[p.q_col[q]] = (float)(1.0 / sqrt(d));
         if (dk_slot) dk_slot[slot] = (float)sqrt(d);
         if (knn64)
             for (int j = 0; j < k; ++j) knn64[slot * k + j] = p.col_ret[knn_col[(int64_t)j * p.nq + q]];
@@ -644,10 +646,10 @@ hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode,
 
 hipError_t launch_knn_finish_list(hipStream_t st, const DevPlan& p, const int* list, const int* n_list, int nq_max,
                                   const int* knn_col, const double* dk2, float* rho_slot, float* dk_slot,
-                                  int64_t* knn64) {
+                                  int64_t* knn64, float* rho_col) {
     if (nq_max <= 0) return hipSuccess;
     const int grid = (nq_max + 255) / 256 < 1024 ? (nq_max + 255) / 256 : 1024;
-    k_knn_finish_list<<<grid, 256, 0, st>>>(p, list, n_list, knn_col, dk2, rho_slot, dk_slot, knn64);
+    k_knn_finish_list<<<grid, 256, 0, st>>>(p, list, n_list, knn_col, dk2, rho_slot, dk_slot, knn64, rho_col);
     return hipGetLastError();
 }
 

@@ -22,7 +22,7 @@ hipError_t launch_knn_finish(hipStream_t st, const DevPlan& p, int nq, int mode,
 // mode 0 of launch_knn_finish for the queries list[0 .. *n_list) (at most nq_max)
 hipError_t launch_knn_finish_list(hipStream_t st, const DevPlan& p, const int* list, const int* n_list, int nq_max,
                                   const int* knn_col, const double* dk2, float* rho_slot, float* dk_slot,
-                                  int64_t* knn64);
+                                  int64_t* knn64, float* rho_col = nullptr);
 // out[q * k + j] = float(sqrt(d2)) of the j-th listed neighbour of query q (canonical FP64 distance)
 hipError_t launch_knn_dists(hipStream_t st, const DevPlan& p, int nq, const int* knn_col, float* out);
 hipError_t launch_rho_cols(hipStream_t st, int64_t n_cols, const int* col_key, const float* rho_slot,

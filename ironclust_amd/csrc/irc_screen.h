@@ -69,6 +69,7 @@ struct ScreenArgs {
     // and the exact d^2 of the first IRC_D2HEAD list entries ([slot][n_q]; NaN for queries it does
     // not finish), which the delta pass uses instead of recomputing them
     double* fin_d2head;
+    float* fin_rho_col;      // rho of every query's own column (the delta pass's per-column rho)
 };
 
 int screen_kp(int D);                                  // padded operand width of the FP32/BF16 forms
@@ -98,7 +99,7 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
                                 int* n_spill);
 // the re-rank finishes its queries itself (fin_* set) only for lists of at most 32
 inline bool screen_rerank_finishes(const ScreenArgs& a) {
-    return a.p.k <= 32 && (a.fin_rho || a.fin_knn || a.fin_d2head);
+    return a.p.k <= 32 && (a.fin_rho || a.fin_knn || a.fin_d2head || a.fin_rho_col);
 }
 hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 // largest |x|^2 of every (block, part) column run

@@ -900,6 +900,7 @@ struct FinOut {
     int64_t* knn;
     double* d2head;   // [IRC_D2HEAD][n_q]
     int first;        // the first pass visits every query: it marks the ones it does not finish
+    float* rho_col;   // rho by column, for the queries' own columns
 };
 
 template <int NV, int W>
@@ -1049,6 +1050,7 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
         if (fin.rho || fin.knn) {
             const int slot = done ? p.q_out[q] : 0;
             if (done && fin.rho) fin.rho[slot] = (float)(1.0 / sqrt(last_t));
+            if (done && fin.rho_col) fin.rho_col[qc] = (float)(1.0 / sqrt(last_t));
             if (done && fin.dk) fin.dk[slot] = (float)sqrt(last_t);
             if (fin.knn) {
                 if (done)
@@ -1353,8 +1355,8 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
     const unsigned grid = (unsigned)((a.n_q + 63) / 64);
     const unsigned grid2 = grid < 1024 ? grid : 1024;   // the second pass takes few queries
     const size_t obytes = a.p.k <= 32 ? sizeof(int) * 64 * (size_t)a.p.k : 0;
-    FinOut fin{nullptr, nullptr, nullptr, nullptr, 1};
-    if (screen_rerank_finishes(a)) fin = FinOut{a.fin_rho, a.fin_dk, a.fin_knn, a.fin_d2head, 1};
+    FinOut fin{nullptr, nullptr, nullptr, nullptr, 1, nullptr};
+    if (screen_rerank_finishes(a)) fin = FinOut{a.fin_rho, a.fin_dk, a.fin_knn, a.fin_d2head, 1, a.fin_rho_col};
     FinOut fin2 = fin;
     fin2.first = 0;
 #define IRC_RR(NV)                                                                                         \
