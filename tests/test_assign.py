@@ -133,6 +133,14 @@ def test_chain_jump_result_does_not_depend_on_interleaving():
     for p in (0.0, 0.5, 1.0):
         for _ in range(5):
             assert np.array_equal(_chain_jump_interleaved(lab0, nneigh, rng, p), want)
+    # one path of 1024 spikes to a single peak: with every read stale (a synchronous round) its far
+    # end needs 10 doublings and one more round to take the label, 11 of the 12 rounds launched
+    n = 1024
+    nneigh = np.concatenate([[1], np.arange(1, n)]).astype(np.int64)
+    lab0 = np.zeros(n, np.int32)
+    lab0[0] = 7
+    for p in (0.0, 1.0):
+        assert np.array_equal(_chain_jump_interleaved(lab0, nneigh, rng, p), np.full(n, 7))
     # the nearest-denser forest of a small recording, peaks labelled by delta
     ds = synth.make_dataset(3000, 2, 5, 3, 3, k=8, drift_step=0.3, n_links=2, band=1, seed=77)
     r = O.sort_batch(ds)
