@@ -7,8 +7,10 @@
  *   rho_paged_site_  (matlab/irc2.m:3653-3668): per site, per drift chunk, the
  *                    candidates are the site's spikes in the linked chunks;
  *   knn_cpu_         (matlab/irc2.m:4244-4263): queries in blocks of 1000,
- *                    Euclidean distances in single precision, k smallest
- *                    (pdist2 'Smallest'), d_k = k-th distance;  rho = 1./d_k (:3669)
+ *                    Euclidean distances in single precision, k smallest with
+ *                    their ids (pdist2 'Smallest'), d_k = k-th distance;
+ *                    rho = 1./d_k (:3669); the ids are mapped to global 1-based
+ *                    spike ids and padded with the last one (:3666-3676)
  *   delta_paged_site_(matlab/irc2.m:3703-3725) + delta_cpu_ (matlab/irc2.m:3981-3994):
  *                    d2 = |y|^2 + |x|^2 - 2 x'y in single precision (GEMM form),
  *                    rho2 <= rho1 masked to inf, column min (first occurrence),
@@ -69,7 +71,7 @@ int cpu_sort_batch(int64_t N, int D, int S, int C, int k,
                    const int32_t* site1, const int32_t* site2,
                    const int64_t* lim, const uint8_t* link,
                    const int32_t* site_list, int n_site_list,
-                   float* rho, float* delta, int64_t* nneigh, int nthreads) {
+                   float* rho, float* delta, int64_t* nneigh, int64_t* knn, int nthreads) {
     if (N <= 0 || D <= 0 || S <= 0 || C <= 0 || k <= 0) return 1;
     int32_t* chunk_of = (int32_t*)malloc(sizeof(int32_t) * N);
     for (int c = 0; c < C; ++c)
@@ -98,6 +100,7 @@ int cpu_sort_batch(int64_t N, int D, int S, int C, int k,
         int64_t cap = 0;
         float* XT = NULL; int64_t* ids = NULL; float* dist = NULL;
         float* topd = (float*)malloc(sizeof(float) * (k + 1));
+        int64_t* topi = (int64_t*)malloc(sizeof(int64_t) * (k + 1));
 #pragma omp for schedule(dynamic, 1)
         for (int si = 0; si < ns; ++si) {
             int s = site_list ? site_list[si] - 1 : si;
@@ -129,22 +132,26 @@ int cpu_sort_batch(int64_t N, int D, int S, int C, int k,
                             const float* xr = XT + d * cap;
                             for (int64_t j = 0; j < m; ++j) { float t = qd - xr[j]; dist[j] += t * t; }
                         }
-                        /* k smallest (pdist2 'Smallest'): sorted insertion with threshold */
+                        /* k smallest with their indices (pdist2 'Smallest'): sorted insertion with threshold */
                         int nt = 0;
                         for (int64_t j = 0; j < m; ++j) {
                             float v = dist[j];
                             if (nt == kk && !(v < topd[kk - 1])) continue;
                             int p = nt < kk ? nt++ : kk - 1;
-                            while (p > 0 && topd[p - 1] > v) { topd[p] = topd[p - 1]; --p; }
+                            while (p > 0 && topd[p - 1] > v) { topd[p] = topd[p - 1]; topi[p] = topi[p - 1]; --p; }
                             topd[p] = v;
+                            topi[p] = j;
                         }
                         float dk = sqrtf(topd[kk - 1]);
                         rho[a] = 1.0f / dk;
+                        /* global ids, padded with the last entry (rho_paged_site_, matlab/irc2.m:3666-3676) */
+                        if (knn)
+                            for (int i = 0; i < k; ++i) knn[a * k + i] = ids[topi[i < kk ? i : kk - 1]] + 1;
                     }
                 }
             }
         }
-        free(XT); free(ids); free(dist); free(topd);
+        free(XT); free(ids); free(dist); free(topd); free(topi);
     }
     /* pass 2: delta_cpu_ per (site, chunk) */
 #pragma omp parallel

@@ -192,14 +192,15 @@ def cpu_port():
     lib = _load("libcpuport.so")
     lib.cpu_sort_batch.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _f32, _f32, _i32, _i32, _i64, _u8, _i32, ctypes.c_int, _f32, _f32, _i64,
-                                   ctypes.c_int]
+                                   _i64, ctypes.c_int]
     lib.cpu_max_threads.restype = ctypes.c_int
     return lib
 
 
 def cpu_sort_batch(ds, sites=None, nthreads=0):
     """FP32 restatement of the reference CPU path (knn_cpu_ + delta_cpu_), optionally on a
-    subset of sites (1-based).  Returns dict(rho, delta, nneigh) (entries of other sites are 0)."""
+    subset of sites (1-based).  Returns dict(rho, delta, nneigh, knn) (entries of other sites are 0;
+    knn is N x k global 1-based ids in ascending FP32 distance)."""
     lib = cpu_port()
     N, D, S, C, k = ds["n_spikes"], ds["D"], ds["n_sites"], ds["n_drift"], ds["k"]
     F1 = _np(ds["F1"], np.float32)
@@ -212,8 +213,9 @@ def cpu_sort_batch(ds, sites=None, nthreads=0):
     rho = np.zeros(N, np.float32)
     delta = np.zeros(N, np.float32)
     nn = np.zeros(N, np.int64)
+    knn = np.zeros((N, k), np.int64)
     st = lib.cpu_sort_batch(N, D, S, C, k, _p(F1, _f32), _p(F2, _f32), _p(s1, _i32), _p(s2, _i32), _p(lim, _i64),
                             _p(link, _u8), _p(sl, _i32), 0 if sl is None else len(sl), _p(rho, _f32),
-                            _p(delta, _f32), _p(nn, _i64), nthreads)
+                            _p(delta, _f32), _p(nn, _i64), _p(knn, _i64), nthreads)
     assert st == 0
-    return dict(rho=rho, delta=delta, nneigh=nn)
+    return dict(rho=rho, delta=delta, nneigh=nn, knn=knn)

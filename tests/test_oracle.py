@@ -259,6 +259,15 @@ def test_cpu_port_agrees_with_oracle():
     assert np.mean(c["nneigh"] == r["nneigh"]) > 0.995
     ok = c["nneigh"] == r["nneigh"]
     np.testing.assert_allclose(c["delta"][ok], r["delta"][ok], rtol=1e-3)
+    # the kNN ids: the same lists up to FP32 near-ties (self first wherever it has no duplicate)
+    same = (c["knn"] == r["knn"]).all(axis=1)
+    assert np.mean(same) > 0.999
+    assert (c["knn"][:, 0] == np.arange(1, ds["n_spikes"] + 1)).mean() > 0.999
+    # the k-th id lies at d_k = 1/rho from its query (primary-site features only here)
+    assert ds["F2"] is None
+    F = np.asarray(ds["F1"], np.float64)
+    d = F[c["knn"][:, -1] - 1] - F
+    np.testing.assert_allclose(np.sqrt((d * d).sum(1)), 1.0 / c["rho"].astype(np.float64), rtol=1e-4)
 
 
 def test_golden_fixtures_reproduce():
