@@ -18,8 +18,18 @@
  *   [vrKnn, miKnn]          = irc_hip_mex('knn_ab', mrFet2, mrFet1, knn)   replaces cuda_knn_ (irc2.m:4202)
  *   [mrKnn, miKnn]          = irc_hip_mex('knn_ab_full', mrFet2, mrFet1, knn)   the cuda_knn_full form
  *   [vrKnn, miKnn]          = irc_hip_mex('knn_ab_ib', mrFet2, viB, mrFet1, knn) the cuda_knn_ib form
- *   irc_hip_mex('reset')   destroys the persistent context
- * Optional trailing argument: precision (0 exact, 1 FP32 screen, 2 BF16 screen, 3 auto).
+ *   [vrOpt, mrTrial]        = irc_hip_mex('autotune', mrFet, viSpk_col, viSite_ptr, vnPrimary_site,
+ *                              viLim_drift, mlDrift, knn [, vrOpt0])
+ *                              times rho_delta_page's call with each candidate setting (irc_autotune)
+ *                              and returns the fastest that reproduces vrOpt0's outputs bit for bit;
+ *                              the context keeps it for later calls with the autotune flag (8).
+ *                              mrTrial has one row per trial: [k_over, waves_per_cu, tile,
+ *                              large_ncand, ms, identical, chosen]
+ *   irc_hip_mex('reset')   destroys the persistent context (and its tuned settings)
+ * Optional trailing argument vrOpt = [precision, flags, k_over, tile, waves_per_cu, large_ncand], any
+ * leading part of it (a scalar is the precision): precision 0 exact, 1 FP32 screen, 2 BF16 screen,
+ * 3 auto, 4 FP8 screen; flags 4 graph replay, 8 autotuned settings, 16 per-site launches; the other
+ * entries as irc_options (0 = the library's default).
  */
 #include <stdint.h>
 #include <string.h>
@@ -91,7 +101,50 @@ static uint8_t* to_u8(const mxArray* a) {
 
 static void opts_from(irc_options* o, int nrhs, const mxArray* prhs[], int pos) {
     irc_options_default(o);
-    if (nrhs > pos) o->precision = (int32_t)mxGetScalar(prhs[pos]);
+    if (nrhs <= pos) return;
+    const mwSize n = numel(prhs[pos]);
+    need(n >= 1 && n <= 6, "options must be [precision, flags, k_over, tile, waves_per_cu, large_ncand] or a leading part");
+    int32_t* v = to_i32(prhs[pos]);
+    int32_t* f[6] = {&o->precision, &o->flags, &o->k_over, &o->tile, &o->waves_per_cu, &o->large_ncand};
+    for (mwSize i = 0; i < n; ++i) *f[i] = v[i];
+    mxFree(v);
+}
+
+/* rho_delta_page / autotune arguments prhs[1..7] -> irc_batch; the index arrays it converts are
+ * returned in tmp[0..4] for mxFree */
+static void page_batch(int nrhs, const mxArray* prhs[], irc_batch* b, void* tmp[5]) {
+    need(nrhs >= 8 && mxIsSingle(prhs[1]), "bad arguments");
+    const mwSize n_cols = mxGetN(prhs[1]);
+    const mwSize S = numel(prhs[4]);
+    need(S >= 1, "vnPrimary_site is empty");
+    need(numel(prhs[2]) == n_cols, "numel(viSpk_col) must equal size(mrFet, 2)");
+    need(numel(prhs[3]) == S + 1, "numel(viSite_ptr) must equal numel(vnPrimary_site) + 1");
+    need(numel(prhs[5]) >= 2, "viLim_drift needs at least two entries");
+    const mwSize C = numel(prhs[5]) - 1;
+    need(mxGetM(prhs[6]) == C && mxGetN(prhs[6]) == C, "mlDrift must be n_drift x n_drift");
+    memset(b, 0, sizeof(*b));
+    b->D = (int32_t)mxGetM(prhs[1]);
+    b->n_cols = (int64_t)n_cols;
+    b->F = (const float*)mxGetData(prhs[1]);
+    int64_t* sp = to_i64(prhs[3]);
+    tmp[0] = sp;
+    need(sp[S] == (int64_t)n_cols, "viSite_ptr(end) must equal size(mrFet, 2)");
+    int32_t* spk = to_i32(prhs[2]);
+    int64_t* np = to_i64(prhs[4]);
+    int64_t* lim = to_i64(prhs[5]);
+    uint8_t* lk = to_u8(prhs[6]);
+    tmp[1] = spk; tmp[2] = np; tmp[3] = lim; tmp[4] = lk;
+    b->col_spike = spk;
+    b->site_ptr = sp;
+    b->site_n_primary = np;
+    b->n_sites = (int32_t)S;
+    b->lim_drift = lim;
+    b->n_drift = (int32_t)C;
+    b->link = lk;
+    b->k = (int32_t)mxGetScalar(prhs[7]);
+    need(b->k >= 1, "knn must be positive");
+    b->n_spikes = 0;
+    for (mwSize s = 0; s < S; ++s) b->n_spikes += np[s];
 }
 
 /* contiguous run marked by vl_in -> (in_begin, n_in) */
@@ -154,37 +207,9 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
         return;
     }
     if (!strcmp(cmd, "rho_delta_page")) {
-        need(nrhs >= 8 && mxIsSingle(prhs[1]), "bad arguments");
-        const mwSize n_cols = mxGetN(prhs[1]);
-        const mwSize S = numel(prhs[4]);
-        need(S >= 1, "vnPrimary_site is empty");
-        need(numel(prhs[2]) == n_cols, "numel(viSpk_col) must equal size(mrFet, 2)");
-        need(numel(prhs[3]) == S + 1, "numel(viSite_ptr) must equal numel(vnPrimary_site) + 1");
-        need(numel(prhs[5]) >= 2, "viLim_drift needs at least two entries");
-        const mwSize C = numel(prhs[5]) - 1;
-        need(mxGetM(prhs[6]) == C && mxGetN(prhs[6]) == C, "mlDrift must be n_drift x n_drift");
         irc_batch b;
-        memset(&b, 0, sizeof(b));
-        b.D = (int32_t)mxGetM(prhs[1]);
-        b.n_cols = (int64_t)n_cols;
-        b.F = (const float*)mxGetData(prhs[1]);
-        int64_t* sp = to_i64(prhs[3]);
-        need(sp[S] == (int64_t)n_cols, "viSite_ptr(end) must equal size(mrFet, 2)");
-        int32_t* spk = to_i32(prhs[2]);
-        int64_t* np = to_i64(prhs[4]);
-        int64_t* lim = to_i64(prhs[5]);
-        uint8_t* lk = to_u8(prhs[6]);
-        b.col_spike = spk;
-        b.site_ptr = sp;
-        b.site_n_primary = np;
-        b.n_sites = (int32_t)S;
-        b.lim_drift = lim;
-        b.n_drift = (int32_t)C;
-        b.link = lk;
-        b.k = (int32_t)mxGetScalar(prhs[7]);
-        need(b.k >= 1, "knn must be positive");
-        b.n_spikes = 0;
-        for (mwSize s = 0; s < S; ++s) b.n_spikes += np[s];
+        void* tmp[5] = {0};
+        page_batch(nrhs, prhs, &b, tmp);
         opts_from(&o, nrhs, prhs, 8);
         plhs[0] = mxCreateNumericMatrix((mwSize)b.n_spikes, 1, mxSINGLE_CLASS, mxREAL);
         plhs[1] = mxCreateNumericMatrix((mwSize)b.n_spikes, 1, mxSINGLE_CLASS, mxREAL);
@@ -198,8 +223,42 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
             plhs[3] = mxCreateNumericMatrix((mwSize)b.k, (mwSize)b.n_spikes, mxINT64_CLASS, mxREAL);
             out.knn = (int64_t*)mxGetData(plhs[3]);
         }
-        check(irc_rho_delta_batch(ctx(), &b, &o, &out));
-        mxFree(spk); mxFree(sp); mxFree(np); mxFree(lim); mxFree(lk);
+        const irc_status st = irc_rho_delta_batch(ctx(), &b, &o, &out);
+        for (int i = 0; i < 5; ++i) mxFree(tmp[i]);
+        check(st);
+        return;
+    }
+    if (!strcmp(cmd, "autotune")) {
+        irc_batch b;
+        void* tmp[5] = {0};
+        page_batch(nrhs, prhs, &b, tmp);
+        opts_from(&o, nrhs, prhs, 8);
+        irc_outputs out;
+        memset(&out, 0, sizeof(out));
+        out.rho = (float*)mxMalloc(sizeof(float) * (size_t)b.n_spikes);
+        out.delta = (float*)mxMalloc(sizeof(float) * (size_t)b.n_spikes);
+        out.nneigh = (int64_t*)mxMalloc(sizeof(int64_t) * (size_t)b.n_spikes);
+        irc_options best;
+        irc_tune_report rep;
+        memset(&rep, 0, sizeof(rep));
+        const irc_status st = irc_autotune(ctx(), &b, &o, &out, &best, &rep);
+        for (int i = 0; i < 5; ++i) mxFree(tmp[i]);
+        mxFree(out.rho); mxFree(out.delta); mxFree(out.nneigh);
+        check(st);
+        plhs[0] = mxCreateNumericMatrix(1, 6, mxDOUBLE_CLASS, mxREAL);
+        double* v = mxGetPr(plhs[0]);
+        v[0] = best.precision; v[1] = best.flags; v[2] = best.k_over;
+        v[3] = best.tile; v[4] = best.waves_per_cu; v[5] = best.large_ncand;
+        if (nlhs > 1) {
+            const mwSize n = (mwSize)rep.n_trials;
+            plhs[1] = mxCreateNumericMatrix(n, 7, mxDOUBLE_CLASS, mxREAL);
+            double* t = mxGetPr(plhs[1]);
+            for (mwSize i = 0; i < n; ++i) {
+                t[i] = rep.k_over[i]; t[i + n] = rep.waves_per_cu[i]; t[i + 2 * n] = rep.tile[i];
+                t[i + 3 * n] = rep.large_ncand[i]; t[i + 4 * n] = rep.ms[i]; t[i + 5 * n] = rep.identical[i];
+                t[i + 6 * n] = (int32_t)i == rep.best_trial;
+            }
+        }
         return;
     }
     if (!strcmp(cmd, "knn_ab") || !strcmp(cmd, "knn_ab_full") || !strcmp(cmd, "knn_ab_ib")) {

@@ -41,6 +41,10 @@ def test_rho_delta_page_length_checks():
     assert "viLim_drift" in _raises(bad)
     bad = list(a); bad[3] = np.zeros(0, np.int64)
     _raises(bad)
+    assert "options" in _raises(a + [np.zeros(7)])
+    assert "options" in _raises(a + [np.zeros(0)], "autotune", 2)
+    bad = list(a); bad[1] = a[1][:-1]
+    assert "viSpk_col" in _raises(bad, "autotune", 2)
 
 
 def test_site_call_length_checks():
@@ -85,3 +89,28 @@ def test_rho_delta_page_through_the_gateway():
     np.testing.assert_array_equal(delta.ravel(), ref["delta"])
     np.testing.assert_array_equal(nn.ravel(), ref["nneigh"])
     np.testing.assert_array_equal(knn.T, ref["knn"])
+
+
+@pytest.mark.gpu
+def test_autotune_through_the_gateway():
+    """'autotune' returns the chosen settings and its trials; rho_delta_page with those settings, and
+    with the autotune flag (8) that uses the context's choice, reproduces the oracle."""
+    from ironclust_amd import irc2, synth
+    from oracle import oracle as O
+    ds = synth.make_dataset(6000, 3, 8, 4, 3, k=9, drift_step=0.3, n_links=2, band=1, seed=13, f2=True)
+    lay = irc2.site_layout(ds)
+    args = [np.ascontiguousarray(lay["F"].numpy().T), lay["col_spike"].numpy(), lay["site_ptr"],
+            lay["site_n_primary"], lay["lim_drift"], lay["link"].astype(bool), np.float64(9)]
+    mex("reset", nlhs=0)
+    opt, trials = mex("autotune", *args, np.array([2.0]), nlhs=2)
+    assert opt.shape == (1, 6) and opt[0, 0] == 2 and opt[0, 1] == 0
+    assert trials.shape[1] == 7 and trials.shape[0] >= 2
+    assert trials[:, 6].sum() == 1 and trials[trials[:, 6] == 1, 5] == 1
+    assert (trials[:, 4] > 0).all()
+    ref = O.sort_batch(ds)
+    for o in (opt.ravel(), np.array([2.0, 8.0])):
+        rho, delta, nn, knn = mex("rho_delta_page", *args, o, nlhs=4)
+        np.testing.assert_array_equal(rho.ravel(), ref["rho"])
+        np.testing.assert_array_equal(delta.ravel(), ref["delta"])
+        np.testing.assert_array_equal(nn.ravel(), ref["nneigh"])
+        np.testing.assert_array_equal(knn.T, ref["knn"])
