@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_screen_pack(int64_t n_cols, int D, int 
                                                      float* __restrict__ feat, int DP,
                                                      const int64_t* __restrict__ seg_ptr, int n_seg,
                                                      const float* __restrict__ seg_scale) {
-    __shared__ float srow[PACK_COLS * IRC_DMAX];   // column c's values at c * FS
+    __shared__ __align__(16) float srow[PACK_COLS * IRC_DMAX];   // column c's values at c * FS (float4 stores)
     __shared__ float snorm[PACK_COLS];
     __shared__ float sscale[PACK_COLS];
     const int64_t col0 = (int64_t)blockIdx.x * PACK_COLS;
