@@ -378,8 +378,10 @@ __global__ __launch_bounds__(64) void k_delta_from_knn(DevPlan p, int nq, const 
 // candidates (rho_b > rho_a, usually a few percent for the queries that reach this kernel) are
 // scored: each 64-candidate chunk is filtered on rho and its denser columns are appended to an
 // LDS queue (ballot + prefix count), and the FP64 chain runs once per 64 queued columns, not once
-// per chunk that holds any.  The (d2, id) minimum does not depend on the scoring order.  The
-// query row is held in registers.
+// per chunk that holds any.  The block's candidate runs (up to 64 at a time) are flattened into
+// one position stream by a prefix sum held in LDS, so a step loads the rho of 512 consecutive
+// positions across run boundaries: a few dependent load rounds per query instead of one per run.
+// The (d2, id) minimum does not depend on the scoring order.  The query row is held in registers.
 //   mode 0 (batch): delta = float(sqrt(d2)*rho_a) with the sentinel rule; nneigh int64
 //   mode 1 (site) : dmin = float(sqrt(d2)) or +Inf; nneigh32 = col_ret (own when none)
 // The query list is qlist[0 .. *nq_dev) when nq_dev is set, else qlist[0..nq) / identity.
@@ -391,6 +393,8 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
                                                     int64_t* __restrict__ nneigh64,
                                                     int* __restrict__ nneigh32) {
     __shared__ int qbuf[128];   // queued denser columns (fewer than 64 carried + one chunk)
+    __shared__ int s_rb[64];     // a batch of the block's runs: first column
+    __shared__ int s_ro[65];     // and first stream position ([64 or fewer] = the batch's length)
     const int lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const int n = nq_dev ? *nq_dev : nq;
@@ -430,35 +434,59 @@ __global__ __launch_bounds__(64) void k_delta_exact(DevPlan p, const int* __rest
             }
         };
         int nbuf = 0;   // wave-uniform queue length, < 64 between chunks
-        for (int li = p.link_ptr[c]; li < p.link_ptr[c + 1]; ++li) {
-            const int b2 = s * p.C + p.link_idx[li];
-            for (int part = 0; part < 2; ++part) {
-                const int beg = p.run_beg[2 * b2 + part];
-                const int len = p.run_len[2 * b2 + part];
-                // four chunks' rho loads in flight at once, then filtered chunk by chunk
-                for (int j4 = 0; j4 < len; j4 += 256) {
-                    float rj[4];
+        const int lp0 = p.link_ptr[c], ne = 2 * (p.link_ptr[c + 1] - lp0);   // runs: (link, part)
+        for (int e0 = 0; e0 < ne; e0 += 64) {
+            // this batch's runs, one lane each, and their stream offsets (inclusive prefix sum)
+            const int e = e0 + lane;
+            int len = 0, beg = 0;
+            if (e < ne) {
+                const int b2 = s * p.C + p.link_idx[lp0 + (e >> 1)];
+                len = p.run_len[2 * b2 + (e & 1)];
+                beg = p.run_beg[2 * b2 + (e & 1)];
+            }
+            int incl = len;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int j = j4 + 64 * u + lane;
-                        rj[u] = j < len ? rho_col[beg + j] : 0.0f;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += t;
+            }
+            const int total = __shfl(incl, 63, 64);
+            __syncthreads();   // the previous batch's table is no longer read
+            s_rb[lane] = beg;
+            s_ro[lane] = incl - len;
+            if (lane == 63) s_ro[64] = total;
+            __syncthreads();
+            // eight chunks' rho loads in flight at once, then filtered chunk by chunk; each lane's
+            // positions ascend, so its run index only moves forward (empty runs are stepped over)
+            int r = 0;
+            for (int j8 = 0; j8 < total; j8 += 512) {
+                int cj[8];
+                float rj[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = j8 + 64 * u + lane;
+                    cj[u] = -1;
+                    if (j < total) {
+                        while (s_ro[r + 1] <= j) ++r;
+                        cj[u] = s_rb[r] + (j - s_ro[r]);
                     }
+                }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int j = j4 + 64 * u + lane;
-                        const bool den = j < len && rj[u] > ra;
-                        const uint64_t m = __ballot(den);
-                        if (den) qbuf[nbuf + __popcll(m & below)] = beg + j;
-                        nbuf += __popcll(m);
-                        if (nbuf >= 64) {
-                            __syncthreads();
-                            score(64);
-                            const int rest = nbuf - 64;
-                            const int mv = lane < rest ? qbuf[64 + lane] : 0;
-                            __syncthreads();
-                            if (lane < rest) qbuf[lane] = mv;
-                            nbuf = rest;
-                        }
+                for (int u = 0; u < 8; ++u) rj[u] = cj[u] >= 0 ? rho_col[cj[u]] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool den = cj[u] >= 0 && rj[u] > ra;
+                    const uint64_t m = __ballot(den);
+                    if (den) qbuf[nbuf + __popcll(m & below)] = cj[u];
+                    nbuf += __popcll(m);
+                    if (nbuf >= 64) {
+                        __syncthreads();
+                        score(64);
+                        const int rest = nbuf - 64;
+                        const int mv = lane < rest ? qbuf[64 + lane] : 0;
+                        __syncthreads();
+                        if (lane < rest) qbuf[lane] = mv;
+                        nbuf = rest;
                     }
                 }
             }
