@@ -42,7 +42,7 @@ struct Bufs {
         b_out_knn, b_out_dk, b_out_nn32, b_out_knn32, b_ncand, b_d2head;
     DevBuf b_scr[6];   // screen: [0] operand rows, [1] FP8 site scales, [2] norms + run maxima, [3] items,
                        // [4] fallback/spill lists, [5] counters
-    DevBuf b_cand_n, b_cand_col, b_cand_d, b_cand_eps;   // certified candidate lists of the screen
+    DevBuf b_cand_n, b_cand, b_cand_eps;   // certified candidate lists of the screen
     DevBuf b_plan;                                         // device-side work planning
     DevBuf b_drift;                                        // drift linkage scratch
     DevBuf b_assign;                                       // chain labelling scratch
@@ -56,7 +56,7 @@ struct Bufs {
         std::vector<DevBuf*> v{&b_F, &b_colspk, &b_feat, &b_key, &b_ret, &b_runb, &b_runl, &b_lptr, &b_lidx,
                                &b_qcol, &b_qblk, &b_qout, &b_small, &b_flags, &b_knncol, &b_dk2, &b_rhoslot,
                                &b_rhocol, &b_unres, &b_out_delta, &b_out_nn, &b_out_knn, &b_out_dk, &b_out_nn32,
-                               &b_out_knn32, &b_ncand, &b_cand_n, &b_cand_col, &b_cand_d, &b_cand_eps, &b_plan,
+                               &b_out_knn32, &b_ncand, &b_cand_n, &b_cand, &b_cand_eps, &b_plan,
                                &b_drift, &b_assign, &b_siteseg, &b_tune, &b_dstat, &b_d2head};
         for (auto& b : b_scr) v.push_back(&b);
         return v;
@@ -529,8 +529,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     // blocks past large_ncand keep 64 slots when the list is 64 deep in registers
     const int slots = (kprime > 32 && any_large) ? 64 : kprime;
     CHK(ensure(ctx, ctx->bb->b_cand_n, sizeof(int) * (size_t)nq));
-    CHK(ensure(ctx, ctx->bb->b_cand_col, sizeof(int) * (size_t)slots * nq));
-    CHK(ensure(ctx, ctx->bb->b_cand_d, sizeof(float) * (size_t)slots * nq));
+    CHK(ensure(ctx, ctx->bb->b_cand, sizeof(int2) * (size_t)slots * nq));
     CHK(ensure(ctx, ctx->bb->b_cand_eps, sizeof(double) * (size_t)nq));
     if (sp == irc::SCREEN_FP8) {
         if (!dw) {
@@ -559,8 +558,7 @@ irc_status run_knn(irc_ctx* ctx, const DevPlan& p, int nq, int prec, int k_over,
     irc::screen_eta(sp, p.D, KP, &a.eta, &a.eta2, &a.eta3);
     a.n_q = nq;
     a.cand_n = ptr<int>(ctx->bb->b_cand_n);
-    a.cand_col = ptr<int>(ctx->bb->b_cand_col);
-    a.cand_d = ptr<float>(ctx->bb->b_cand_d);
+    a.cand = ptr<int2>(ctx->bb->b_cand);
     a.cand_eps = ptr<double>(ctx->bb->b_cand_eps);
     a.fallback = fb;
     a.n_fallback = nfb;

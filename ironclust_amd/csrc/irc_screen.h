@@ -53,11 +53,11 @@ struct ScreenArgs {
     // |screened - exact| <= eta (|q|^2 + |c|^2) + eta2 s (|q| + |c|) + eta3 s^2
     double eta, eta2, eta3;
     // certified candidate lists, slot-major ([slot][n_q]) so a wavefront's 64 queries write
-    // and the re-rank's lanes read contiguous words
+    // and the re-rank's lanes read contiguous pairs
     int64_t n_q;             // queries of the plan (row stride of the lists)
     int* cand_n;             // per query: r | (min(k, |C|) << 16), or -1 (not screened / uncertified)
-    int* cand_col;           // [slots][n_q] candidate columns in ascending screened order
-    float* cand_d;           // [slots][n_q] their truncated screened distances
+    int2* cand;              // [slots][n_q] (candidate column, bits of its truncated screened
+                             // distance) in ascending screened order
     double* cand_eps;        // per query: the screen's error bound
     int* fallback;           // queries to redo with the exact kernel
     int* n_fallback;

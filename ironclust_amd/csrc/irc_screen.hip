@@ -852,8 +852,8 @@ __global__ __launch_bounds__(64, 2) void k_screen_knn(ScreenArgs a, int* __restr
                     const int32_t key = (int32_t)sbuf[j * 64 + lane];
                     const int pos = (int)(key & pmask);
                     const int lo = use_tbl ? (int)tbl[pos] : run_of(pos);
-                    a.cand_col[(int64_t)j * a.n_q + q_mine] = s_rb[lo] + (pos - s_ro[lo]);
-                    a.cand_d[(int64_t)j * a.n_q + q_mine] = (float)((double)key_dist(key, hmask) * s2);
+                    a.cand[(int64_t)j * a.n_q + q_mine] =
+                        make_int2(s_rb[lo] + (pos - s_ro[lo]), __float_as_int((float)((double)key_dist(key, hmask) * s2)));
                 }
                 a.cand_n[q_mine] = r | (kk << 16);
                 a.cand_eps[q_mine] = eps;
@@ -905,11 +905,11 @@ struct FinOut {
 
 template <int NV, int W>
 __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int64_t q, bool qv,
-                                             const int* __restrict__ cand_n, const int* __restrict__ cand_col,
-                                             const float* __restrict__ cand_d, const double* __restrict__ cand_eps,
+                                             const int* __restrict__ cand_n, const int2* __restrict__ cand,
+                                             const double* __restrict__ cand_eps,
                                              int* __restrict__ knn_col, double* __restrict__ dk2,
                                              int* __restrict__ spill, int* __restrict__ n_spill, float4* stage,
-                                             int* skey, float* sdn, int* obuf, const FinOut& fin) {
+                                             int* skey, int* obuf, const FinOut& fin) {
     const int l = threadIdx.x;
     const int rn = qv ? cand_n[q] : -1;
     const int r = rn < 0 ? 0 : (rn & 0xffff);
@@ -928,16 +928,13 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
 #pragma unroll
         for (int i = 0; i < NV; ++i) qr[i] = q4[i];
     }
-    // lanes past their own list fetch their query's row (a valid address) and ignore it
-    auto slot_col = [&](int j) { return j < r ? cand_col[(int64_t)j * n_q + q] : qc; };
-    // slot j's rows, ids and the next slot's screened distances, straight into LDS
-    auto gather = [&](int j, int colj) {
-        const int jn = j + 1 < r ? j + 1 : (r > 0 ? r - 1 : 0);
+    // slot j's (column, screened distance); lanes past their own list fetch their query's row (a
+    // valid address) and ignore it
+    auto slot = [&](int j) { return j < r ? cand[(int64_t)j * n_q + q] : make_int2(qc, 0); };
+    // slot j's rows and ids, straight into LDS
+    auto gather = [&](int colj) {
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(p.col_key + colj),
                                          (__attribute__((address_space(3))) void*)skey, 4, 0, 0);
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(cand_d + (int64_t)jn * n_q + (qv ? q : 0)),
-            (__attribute__((address_space(3))) void*)sdn, 4, 0, 0);
         int crow[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) crow[i] = __shfl(colj, (i * 64 + l) / NV, 64);
@@ -960,20 +957,21 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
     double last_t = 0.0;
     int last_c = -1;
     bool ok = rn >= 0;
-    int colj = slot_col(0);
-    int coln = slot_col(1);
-    gather(0, colj);
+    int colj = slot(0).x;
+    int2 sn = slot(1);
+    gather(colj);
     for (int j = 0; j < rmax; ++j) {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the gather of slot j has landed in LDS
         float4 b[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) b[i] = stage[l * NV + i];
         const int kc = skey[l];
-        const float dn = sdn[l];
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");   // LDS read before the next gather lands
         const bool act = ok && j < r && out < kk;
-        const int coln2 = slot_col(j + 2);
-        if (j + 1 < rmax) gather(j + 1, coln);
+        const int coln = sn.x;
+        const float dn = __int_as_float(sn.y);   // the next slot's screened distance
+        sn = slot(j + 2);
+        if (j + 1 < rmax) gather(coln);
         if (act) {
 #pragma unroll
             for (int i = 0; i < NV; ++i)
@@ -1021,7 +1019,6 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
             }
         }
         colj = coln;
-        coln = coln2;
     }
     const bool done = qv && rn >= 0 && ok && out == kk;
     if (done) {
@@ -1070,13 +1067,12 @@ __device__ __forceinline__ void rerank_group(const DevPlan& p, int64_t n_q, int6
 
 template <int NV>
 __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int* __restrict__ cand_n,
-                                              const int* __restrict__ cand_col, const float* __restrict__ cand_d,
+                                              const int2* __restrict__ cand,
                                               const double* __restrict__ cand_eps, int* __restrict__ knn_col,
                                               double* __restrict__ dk2, int* __restrict__ spill,
                                               int* __restrict__ n_spill, FinOut fin) {
     __shared__ float4 stage[64 * NV];   // [row][piece]: the slot's 64 candidate rows
     __shared__ int skey[64];            // their spike ids
-    __shared__ float sdn[64];           // each lane's next screened distance (lower bounds)
     extern __shared__ int obuf[];       // [k][64] emitted columns when k <= 32 (else written directly)
     // workgroups are dispatched round-robin over the 8 XCDs: give each XCD a contiguous eighth
     // of the query groups, so the candidate rows its groups share stay in its L2
@@ -1084,28 +1080,27 @@ __global__ __launch_bounds__(64) void k_rerank(DevPlan p, int64_t n_q, const int
     const unsigned grp = (lo8 == 0 || x < lo8) ? x * per + (blockIdx.x >> 3)
                                                : lo8 * per + (x - lo8) * (per - 1) + (blockIdx.x >> 3);
     const int64_t q = (int64_t)grp * 64 + threadIdx.x;
-    rerank_group<NV, WIN1>(p, n_q, q, q < n_q, cand_n, cand_col, cand_d, cand_eps, knn_col, dk2, spill, n_spill,
-                           stage, skey, sdn, obuf, fin);
+    rerank_group<NV, WIN1>(p, n_q, q, q < n_q, cand_n, cand, cand_eps, knn_col, dk2, spill, n_spill, stage, skey,
+                           obuf, fin);
 }
 
 // second pass: the queries of list[0 .. *n_list), 64 per group, groups strided over the grid
 template <int NV>
 __global__ __launch_bounds__(64) void k_rerank_list(DevPlan p, int64_t n_q, const int* __restrict__ list,
                                                    const int* __restrict__ n_list, const int* __restrict__ cand_n,
-                                                   const int* __restrict__ cand_col, const float* __restrict__ cand_d,
+                                                   const int2* __restrict__ cand,
                                                    const double* __restrict__ cand_eps, int* __restrict__ knn_col,
                                                    double* __restrict__ dk2, int* __restrict__ fallback,
                                                    int* __restrict__ n_fallback, FinOut fin) {
     __shared__ float4 stage[64 * NV];
     __shared__ int skey[64];
-    __shared__ float sdn[64];
     extern __shared__ int obuf[];
     const int64_t n = *n_list;
     for (int64_t g = blockIdx.x; g * 64 < n; g += gridDim.x) {
         const int64_t i = g * 64 + threadIdx.x;
         const bool qv = i < n;
-        rerank_group<NV, WIN2>(p, n_q, qv ? (int64_t)list[i] : 0, qv, cand_n, cand_col, cand_d, cand_eps, knn_col,
-                               dk2, fallback, n_fallback, stage, skey, sdn, obuf, fin);
+        rerank_group<NV, WIN2>(p, n_q, qv ? (int64_t)list[i] : 0, qv, cand_n, cand, cand_eps, knn_col, dk2,
+                               fallback, n_fallback, stage, skey, obuf, fin);
     }
 }
 
@@ -1361,10 +1356,10 @@ hipError_t launch_screen_rerank(hipStream_t st, const ScreenArgs& a, int* knn_co
     fin2.first = 0;
 #define IRC_RR(NV)                                                                                         \
     case NV:                                                                                               \
-        k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand_col, a.cand_d, a.cand_eps, knn_col, \
-                                          dk2, spill, n_spill, fin);                                       \
-        k_rerank_list<NV><<<grid2, 64, obytes, st>>>(a.p, a.n_q, spill, n_spill, a.cand_n, a.cand_col, a.cand_d, \
-                                                a.cand_eps, knn_col, dk2, a.fallback, a.n_fallback, fin2); \
+        k_rerank<NV><<<grid, 64, obytes, st>>>(a.p, a.n_q, a.cand_n, a.cand, a.cand_eps, knn_col, dk2, spill, \
+                                          n_spill, fin);                                                   \
+        k_rerank_list<NV><<<grid2, 64, obytes, st>>>(a.p, a.n_q, spill, n_spill, a.cand_n, a.cand, a.cand_eps, \
+                                                knn_col, dk2, a.fallback, a.n_fallback, fin2);             \
         break;
     switch (a.p.DP / 4) {
         IRC_RR(1) IRC_RR(2) IRC_RR(3) IRC_RR(4) IRC_RR(5) IRC_RR(6) IRC_RR(7) IRC_RR(8)
