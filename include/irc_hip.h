@@ -250,6 +250,14 @@ irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st);
  * the screen kernel relies on (no reference counterpart: a diagnostic).  out: 6144 floats, host memory. */
 irc_status irc_selftest_mfma(irc_ctx* ctx, float* out);
 
+/* Accumulation self-test of the screen's MFMA chains (no reference counterpart: a diagnostic of the
+ * rounding model the certified screen bound assumes).  For t < n_tiles: out[t] (32 x 32) = A[t]
+ * (32 x K) times B[t] (K x 32), all row-major float, host memory, computed by one chain of
+ * v_mfma_f32_32x32x2_f32 (prec 1, K even) or of v_mfma_f32_32x32x16_bf16 (prec 2, K a multiple of
+ * 16; the entries of A and B must be representable in bf16).  2 <= K <= 64, 1 <= n_tiles <= 65536. */
+irc_status irc_selftest_mfma_accum(irc_ctx* ctx, int prec, int K, int n_tiles, const float* A, const float* B,
+                                   float* out);
+
 /* ---------------------------------------------------------------------------
  * Legacy A/B forms (irc v4 path).
  * irc_knn_ab: B = candidates (D x nB), A = queries (D x nA).  K_out[nA] = sqrt(d2_k);

@@ -770,6 +770,29 @@ irc_status irc_selftest_mfma(irc_ctx* ctx, float* out) {
     return IRC_OK;
 }
 
+irc_status irc_selftest_mfma_accum(irc_ctx* ctx, int prec, int K, int n_tiles, const float* A, const float* B,
+                                   float* out) {
+    if (!ctx) return IRC_E_ARG;
+    if (!A || !B || !out) return fail(ctx, IRC_E_ARG, "null pointer");
+    if (prec != 1 && prec != 2) return fail(ctx, IRC_E_ARG, "prec must be 1 (FP32 MFMA) or 2 (BF16 MFMA)");
+    if (K < 2 || K > 64 || K % (prec == 1 ? 2 : 16) != 0)
+        return fail(ctx, IRC_E_ARG, "K=%d: 2..64, even (prec 1) or a multiple of 16 (prec 2)", K);
+    if (n_tiles < 1 || n_tiles > 65536) return fail(ctx, IRC_E_ARG, "n_tiles=%d outside 1..65536", n_tiles);
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t na = (size_t)n_tiles * 32 * K, no = (size_t)n_tiles * 1024;
+    CHK(ensure(ctx, ctx->bb->b_scr[0], sizeof(float) * (2 * na + no)));
+    float* dA = ptr<float>(ctx->bb->b_scr[0]);
+    float* dB = dA + na;
+    float* dO = dB + na;
+    hipStream_t st = ctx->stream;
+    HIPCHK(hipMemcpyAsync(dA, A, sizeof(float) * na, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dB, B, sizeof(float) * na, hipMemcpyHostToDevice, st));
+    HIPCHK(irc::launch_mfma_accum_probe(st, prec, K, n_tiles, dA, dB, dO));
+    HIPCHK(hipMemcpyAsync(out, dO, sizeof(float) * no, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return IRC_OK;
+}
+
 irc_status irc_get_stats(const irc_ctx* ctx, irc_stats* st) {
     if (!ctx || !st) return IRC_E_ARG;
     *st = ctx->stats;

@@ -106,5 +106,7 @@ hipError_t launch_fill_int(hipStream_t st, int* p, int64_t n, int v);
 hipError_t launch_run_nmax(hipStream_t st, int64_t n_runs, const int* run_beg, const int* run_len, const float* norm,
                            float* out);
 hipError_t launch_mfma_layout_probe(hipStream_t st, float* out);
+hipError_t launch_mfma_accum_probe(hipStream_t st, int prec, int K, int n_tiles, const float* A, const float* B,
+                                   float* out);
 
 }  // namespace irc

@@ -1388,4 +1388,42 @@ hipError_t launch_mfma_layout_probe(hipStream_t st, float* out) {
     return hipGetLastError();
 }
 
+// Accumulation probe: tile t's A_t (32 x K, row-major) times B_t (K x 32, row-major) by one chain of
+// the MFMAs the screen uses -- prec 1: v_mfma_f32_32x32x2_f32, two k per step; prec 2:
+// v_mfma_f32_32x32x16_bf16, sixteen k per step, the operands being floats that bf16 represents
+// exactly (their upper halves are passed) -- written as out[t][row][col] through the 32 x 32
+// accumulator layout (register r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32).
+__global__ __launch_bounds__(64) void k_mfma_accum_probe(int prec, int K, const float* __restrict__ A,
+                                                         const float* __restrict__ B, float* __restrict__ out) {
+    const int l = threadIdx.x, h = l >> 5, m = l & 31;
+    const int64_t t = blockIdx.x;
+    const float* a = A + t * 32 * K;
+    const float* b = B + t * K * 32;
+    f32x16 acc = {};
+    if (prec == 1) {
+        for (int k0 = 0; k0 < K; k0 += 2)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[m * K + k0 + h], b[(k0 + h) * 32 + m], acc, 0, 0, 0);
+    } else {
+        for (int k0 = 0; k0 < K; k0 += 16) {
+            short8 va, vb;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + 8 * h + j;
+                va[j] = (short)(__float_as_uint(a[m * K + k]) >> 16);
+                vb[j] = (short)(__float_as_uint(b[k * 32 + m]) >> 16);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, va),
+                                                          __builtin_bit_cast(bf16x8, vb), acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) out[t * 1024 + (8 * (r >> 2) + 4 * h + (r & 3)) * 32 + m] = acc[r];
+}
+
+hipError_t launch_mfma_accum_probe(hipStream_t st, int prec, int K, int n_tiles, const float* A, const float* B,
+                                   float* out) {
+    k_mfma_accum_probe<<<n_tiles, 64, 0, st>>>(prec, K, A, B, out);
+    return hipGetLastError();
+}
+
 }  // namespace irc

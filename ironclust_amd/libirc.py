@@ -19,7 +19,7 @@ FLAG_GRAPH, FLAG_AUTOTUNE, FLAG_SITE_LAUNCHES = 4, 8, 16          # irc_options.
 EXPORTED = [
     "irc_options_default", "irc_ctx_create", "irc_ctx_destroy", "irc_last_error", "irc_version",
     "irc_knn_drift", "irc_delta_drift", "irc_rho_delta_batch", "irc_get_stats",
-    "irc_knn_ab", "irc_knn_ab_full", "irc_knn_ab_ib", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma",
+    "irc_knn_ab", "irc_knn_ab_full", "irc_knn_ab_ib", "irc_delta_knn_ab", "irc_rho_drift_dc", "irc_delta_drift_dc", "irc_selftest_mfma", "irc_selftest_mfma_accum",
     "irc_calc_drift", "irc_assign_chain", "irc_isolation_counts", "irc_autotune", "irc_rho_delta_batch_noalloc",
 ]
 
@@ -125,6 +125,9 @@ def load(path=None):
     lib.irc_get_stats.restype = ctypes.c_int
     lib.irc_selftest_mfma.argtypes = [ctypes.c_void_p, _c_f32p]
     lib.irc_selftest_mfma.restype = ctypes.c_int
+    lib.irc_selftest_mfma_accum.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_f32p, _c_f32p,
+                                            _c_f32p]
+    lib.irc_selftest_mfma_accum.restype = ctypes.c_int
     lib.irc_knn_ab.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, _c_f32p, ctypes.c_int64, _c_f32p,
                                ctypes.c_int32, _c_f32p, _c_i32p]
     lib.irc_knn_ab.restype = ctypes.c_int
@@ -218,6 +221,17 @@ class Context:
     def selftest_mfma(self):
         out = np.zeros(6144, np.float32)
         self._check(self.lib.irc_selftest_mfma(self.h, _p(out, _c_f32p)))
+        return out
+
+    def selftest_mfma_accum(self, prec, A, B):
+        """(T, 32, K) x (T, K, 32) float32 tiles through one MFMA chain of the screen's form prec
+        (1 FP32, 2 BF16); returns (T, 32, 32)."""
+        A = np.ascontiguousarray(A, np.float32)
+        B = np.ascontiguousarray(B, np.float32)
+        T, _, K = A.shape
+        out = np.zeros((T, 32, 32), np.float32)
+        self._check(self.lib.irc_selftest_mfma_accum(self.h, prec, K, T, _p(A, _c_f32p), _p(B, _c_f32p),
+                                                     _p(out, _c_f32p)))
         return out
 
     # ---- per-site forms (search_knn_drift_ / search_delta_drift_) --------------

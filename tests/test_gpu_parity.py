@@ -385,6 +385,49 @@ def test_mfma_lane_layout(ctx):
     np.testing.assert_array_equal(out[5], want8, err_msg="fp8 32x32x64 layout and e4m3 decoding (half 1)")
 
 
+def _bf16(x):
+    """Round float32 values to the nearest bf16 (ties to even), returned as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("prec,K", [(1, 48), (1, 64), (2, 48), (2, 64)])
+def test_mfma_accumulation_within_the_screen_bound(ctx, prec, K):
+    """The certified screen bound (screen_eta) models every FP32 accumulation inside the matrix core
+    with unit roundoff 2^-23, twice FP32's.  Measured here on the screen's own MFMA forms (FP32
+    32x32x2 chains; BF16 32x32x16 chains, whose bf16 products are exact): every output of random tiles
+    with rows and columns scaled over 2^+-12, of tiles whose products cancel exactly, and of tiles
+    with entries scaled over 2^+-20 one by one stays within gamma_{K+2}(2^-23) * sum |a_i b_i| of the
+    exact sum."""
+    rng = np.random.default_rng(100 + 10 * prec + K)
+    T = 384
+    A = rng.standard_normal((T, 32, K)) * 2.0 ** rng.integers(-12, 13, size=(T, 32, 1))
+    B = rng.standard_normal((T, K, 32)) * 2.0 ** rng.integers(-12, 13, size=(T, 1, 32))
+    c = slice(128, 256)   # exact cancellation: the second half of k repeats the first with B negated
+    A[c, :, K // 2:] = A[c, :, :K // 2]
+    B[c, K // 2:, :] = -B[c, :K // 2, :]
+    w = slice(256, T)     # entries scaled one by one
+    A[w] *= 2.0 ** rng.integers(-20, 21, size=A[w].shape)
+    B[w] *= 2.0 ** rng.integers(-20, 21, size=B[w].shape)
+    A = A.astype(np.float32)
+    B = B.astype(np.float32)
+    if prec == 2:
+        A, B = _bf16(A), _bf16(B)
+    out = ctx.selftest_mfma_accum(prec, A, B).astype(np.float64)
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    exact = np.matmul(A64, B64)
+    scale = np.matmul(np.abs(A64), np.abs(B64))
+    u = 2.0 ** -23
+    gamma = (K + 2) * u / (1 - (K + 2) * u)
+    err = np.abs(out - exact)
+    ratio = err / (u * scale)
+    print(f"prec {prec} K {K}: largest error {ratio.max():.3f} u * sum|ab| (bound {gamma / u:.1f}); "
+          f"cancelling tiles: largest |result| {err[c].max() / scale[c].max():.2e} of sum|ab|")
+    assert (err <= gamma * scale).all(), ratio.max()
+    assert np.abs(out[c]).max() <= gamma * scale[c].max()
+
+
 def _dev_batch(ds, k):
     import torch
     lay = irc2.site_layout(ds)
