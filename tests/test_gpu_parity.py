@@ -428,6 +428,19 @@ def test_mfma_accumulation_within_the_screen_bound(ctx, prec, K):
     assert np.abs(out[c]).max() <= gamma * scale[c].max()
 
 
+def test_mfma_accum_selftest_rejects_bad_shapes(ctx):
+    A = np.ones((2, 32, 47), np.float32)
+    B = np.ones((2, 47, 32), np.float32)
+    with pytest.raises(libirc.IrcError):
+        ctx.selftest_mfma_accum(1, A, B)                      # odd K for the FP32 form
+    A, B = np.ones((2, 32, 40), np.float32), np.ones((2, 40, 32), np.float32)
+    with pytest.raises(libirc.IrcError):
+        ctx.selftest_mfma_accum(2, A, B)                      # K not a multiple of 16 for BF16
+    with pytest.raises(libirc.IrcError):
+        ctx.selftest_mfma_accum(3, A, B)                      # no such form
+    np.testing.assert_array_equal(ctx.selftest_mfma_accum(1, A, B), np.full((2, 32, 32), 40.0, np.float32))
+
+
 def _dev_batch(ds, k):
     import torch
     lay = irc2.site_layout(ds)
